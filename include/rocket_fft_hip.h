@@ -1,0 +1,136 @@
+/*
+ * rocket_fft_hip.h -- C ABI of the MI355X (gfx950) transform engine.
+ *
+ * The ten numba_* entry points keep the names, argument order and meaning of
+ * rocket-fft's own C ABI (rocket_fft/_pocketfft_numba.cpp:25-223), which
+ * rocket-fft's Numba intrinsics bind by symbol name with the LLVM prototypes
+ * of rocket_fft/pocketfft.py:32-128 (i64 for integers, i1 for bools, double
+ * for fct, i8* for arrays).  Arrays are Numba array structs: shape first,
+ * then strides in BYTES (numba _arraystruct.h).
+ *
+ * Memory: data pointers may be device memory (hipMalloc / torch CUDA
+ * tensors: computed in place on the GPU) or ordinary host memory (staged
+ * through device buffers, then copied back; correctness path).
+ *
+ * Semantics follow the reference engine exactly: unnormalised transforms
+ * times fct, fct applied once (first axis), zero-size arrays are a no-op,
+ * in-place (ain == aout) allowed for c2c, dct, dst, r2r_*.  Like the
+ * reference, the functions return nothing; a failure (invalid DCT type,
+ * HIP error, unsupported geometry) leaves aout unspecified, prints one line
+ * to stderr and is reported by rfh_last_error() on the calling thread.
+ *
+ * By default each call completes before it returns (the reference is
+ * synchronous).  rfh_set_stream() lets a thread run its calls
+ * asynchronously on its own HIP stream instead.
+ */
+#ifndef ROCKET_FFT_HIP_H
+#define ROCKET_FFT_HIP_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Numba's array struct (numba/_arraystruct.h); shape_and_strides holds
+ * 2*ndim entries: the shape, then the strides in bytes. */
+typedef struct {
+    void *meminfo;
+    void *parent;
+    int64_t nitems;
+    int64_t itemsize;
+    void *data;
+    int64_t shape_and_strides[];
+} arystruct_t;
+
+/* ---- drop-in entry points (rocket_fft/_pocketfft_numba.cpp) ---------------- */
+
+/* _pocketfft_numba.cpp:25-29: smallest 2^a3^b5^c7^d11^e >= target (real:
+ * 2^a3^b5^c), target itself below 13 (real: below 7).  Bit-exact. */
+uint64_t numba_good_size(uint64_t target, bool real);
+
+/* _pocketfft_numba.cpp:31-49: complex -> complex over axes; itemsize 16 of
+ * ain selects complex128, else complex64. */
+void numba_c2c(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout, arystruct_t *axes,
+               bool forward, double fct, uint64_t nthreads);
+
+/* _pocketfft_numba.cpp:51-69: DCT type 1..4 over axes; itemsize 8 -> float64. */
+void numba_dct(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout, arystruct_t *axes,
+               uint64_t type, double fct, bool ortho, uint64_t nthreads);
+
+/* _pocketfft_numba.cpp:71-89: DST type 1..4 over axes. */
+void numba_dst(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout, arystruct_t *axes,
+               uint64_t type, double fct, bool ortho, uint64_t nthreads);
+
+/* _pocketfft_numba.cpp:91-109: real -> half complex (last axis n/2+1),
+ * shape taken from ain; conjugated output when !forward. */
+void numba_r2c(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout, arystruct_t *axes,
+               bool forward, double fct, uint64_t nthreads);
+
+/* _pocketfft_numba.cpp:111-143: r2c into a full-size complex aout, then the
+ * Hermitian mirror fills the other half. */
+void numba_c2c_sym(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout,
+                   arystruct_t *axes, bool forward, double fct, uint64_t nthreads);
+
+/* _pocketfft_numba.cpp:145-163: half complex -> real; shape taken from aout. */
+void numba_c2r(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout, arystruct_t *axes,
+               bool forward, double fct, uint64_t nthreads);
+
+/* _pocketfft_numba.cpp:165-183: FFTPACK halfcomplex real transforms. */
+void numba_r2r_fftpack(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout,
+                       arystruct_t *axes, bool real2hermitian, bool forward, double fct,
+                       uint64_t nthreads);
+
+/* _pocketfft_numba.cpp:185-203: Hartley transform, axis by axis. */
+void numba_r2r_separable_hartley(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout,
+                                 arystruct_t *axes, double fct, uint64_t nthreads);
+
+/* _pocketfft_numba.cpp:205-223: true N-D Hartley transform. */
+void numba_r2r_genuine_hartley(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout,
+                               arystruct_t *axes, double fct, uint64_t nthreads);
+
+/* ---- additions without a reference counterpart ------------------------------ */
+
+/* Last error of the calling thread: 0 = none.  Copies the message (NUL
+ * terminated, truncated to len) when buf is non-null.  rfh_clear_error()
+ * resets it.  For tests and callers that want to check the void entry
+ * points above. */
+int rfh_last_error(char *buf, size_t len);
+void rfh_clear_error(void);
+
+/* Run this thread's subsequent calls on `stream` (a hipStream_t; NULL = the
+ * default stream).  blocking = true (the default) makes every call wait for
+ * its work; false returns after enqueueing (device pointers only; host
+ * arrays always block).  Returns 0 on success. */
+int rfh_set_stream(void *stream, bool blocking);
+
+/* Enables (default) or disables the compile-time specialised line kernels
+ * for power-of-two lengths; disabled, every transform runs the generic
+ * kernels.  Process-wide; meant for tests that compare the two paths. */
+int rfh_set_fast_path(bool enabled);
+
+/* Plan facts of the reference engine, restated bit-exactly:
+ * complex FFTPACK factor list (_pocketfft_hdronly.h:1750-1778), real FFTPACK
+ * factor list (:2629-2652), and the FFTPACK-vs-Bluestein decision
+ * (:2843-2861 complex, :2885-2903 real) with its Bluestein length n2
+ * (:2765-2767).  factors return the count (entries beyond cap dropped);
+ * plan_choice returns 1 for Bluestein (n2 written) and 0 for FFTPACK. */
+int rfh_cfftp_factors(uint64_t n, uint64_t *out, int cap);
+int rfh_rfftp_factors(uint64_t n, uint64_t *out, int cap);
+int rfh_plan_choice(uint64_t n, bool real, uint64_t *n2);
+
+/* GPU execution schedule chosen for a complex transform of length n
+ * (double_precision selects complex128): kind 0 = single pass in LDS,
+ * 1 = Bluestein in LDS, 2 = two-pass (four-step), 3 = Bluestein two-pass;
+ * radices of the (first) LDS pass in execution order; n1, n2 = the
+ * two-pass split (0 when single pass).  Returns the number of radices. */
+int rfh_gpu_schedule(uint64_t n, bool double_precision, int *kind, int *radices, int cap,
+                     uint64_t *n1, uint64_t *n2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ROCKET_FFT_HIP_H */

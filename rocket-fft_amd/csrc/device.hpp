@@ -1,0 +1,380 @@
+// device.hpp -- device-side building blocks shared by the line kernels:
+// complex arithmetic, radix butterflies, Stockham stages, the kernel
+// argument block and the load/store helpers.
+#pragma once
+
+#include "rfh.hpp"
+
+namespace rfh {
+namespace dev {
+
+// ------------------------------------------------------------ complex math
+
+template <typename T>
+struct alignas(2 * sizeof(T)) C2 {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ C2<T> operator+(C2<T> a, C2<T> b) {
+  return {a.x + b.x, a.y + b.y};
+}
+template <typename T>
+__device__ __forceinline__ C2<T> operator-(C2<T> a, C2<T> b) {
+  return {a.x - b.x, a.y - b.y};
+}
+template <typename T>
+__device__ __forceinline__ C2<T> operator*(T s, C2<T> a) {
+  return {s * a.x, s * a.y};
+}
+template <typename T>
+__device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ C2<T> conjc(C2<T> a) {
+  return {a.x, -a.y};
+}
+// a * (dir * i)
+template <typename T>
+__device__ __forceinline__ C2<T> rot(C2<T> a, T dir) {
+  return {-dir * a.y, dir * a.x};
+}
+// w, conjugated when dir > 0 (tables hold forward-sign roots)
+template <typename T>
+__device__ __forceinline__ C2<T> dirw(C2<T> w, T dir) {
+  return {w.x, dir < T(0) ? w.y : -w.y};
+}
+
+// q = u / d for 0 <= u < 2^22 via a float reciprocal and one correction.
+__device__ __forceinline__ int fdiv(int u, int d, float inv) {
+  int q = int(float(u) * inv);
+  const int r = u - q * d;
+  q += (r >= d) - (r < 0);
+  return q;
+}
+
+// -------------------------------------------------------------- butterflies
+// dir = -1: X_k = sum_m v_m e^{-2 pi i m k / R}; dir = +1 the conjugate kernel.
+
+template <typename T>
+__device__ __forceinline__ void dft2(C2<T> *v) {
+  const C2<T> a = v[0];
+  v[0] = a + v[1];
+  v[1] = a - v[1];
+}
+
+template <typename T>
+__device__ __forceinline__ void dft3(C2<T> *v, T dir) {
+  const T s = T(0.866025403784438646763723170752936183);
+  const C2<T> t = v[1] + v[2], d = v[1] - v[2];
+  const C2<T> a = v[0] + T(-0.5) * t;
+  v[0] = v[0] + t;
+  const C2<T> b = rot(s * d, dir);
+  v[1] = a + b;
+  v[2] = a - b;
+}
+
+template <typename T>
+__device__ __forceinline__ void dft4(C2<T> *v, T dir) {
+  const C2<T> t0 = v[0] + v[2], t1 = v[0] - v[2], t2 = v[1] + v[3];
+  const C2<T> t3 = rot(v[1] - v[3], dir);
+  v[0] = t0 + t2;
+  v[2] = t0 - t2;
+  v[1] = t1 + t3;
+  v[3] = t1 - t3;
+}
+
+template <typename T>
+__device__ __forceinline__ void dft5(C2<T> *v, T dir) {
+  const T c1 = T(0.309016994374947424102293417182819059);
+  const T c2 = T(-0.809016994374947424102293417182819059);
+  const T s1 = T(0.951056516295153572116439333379382143);
+  const T s2 = T(0.587785252292473129168705954639072769);
+  const C2<T> t1 = v[1] + v[4], t2 = v[2] + v[3], d1 = v[1] - v[4], d2 = v[2] - v[3];
+  const C2<T> a1 = v[0] + c1 * t1 + c2 * t2, a2 = v[0] + c2 * t1 + c1 * t2;
+  const C2<T> b1 = rot(s1 * d1 + s2 * d2, dir), b2 = rot(s2 * d1 - s1 * d2, dir);
+  v[0] = v[0] + t1 + t2;
+  v[1] = a1 + b1;
+  v[4] = a1 - b1;
+  v[2] = a2 + b2;
+  v[3] = a2 - b2;
+}
+
+template <typename T>
+__device__ __forceinline__ void dft8(C2<T> *v, T dir) {
+  C2<T> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+  dft4(e, dir);
+  dft4(o, dir);
+  const T s = T(0.707106781186547524400844362104849039);
+  const C2<T> o1 = {s * (o[1].x - dir * o[1].y), s * (o[1].y + dir * o[1].x)};
+  const C2<T> o2 = rot(o[2], dir);
+  const C2<T> o3 = {s * (-o[3].x - dir * o[3].y), s * (-o[3].y + dir * o[3].x)};
+  v[0] = e[0] + o[0];
+  v[4] = e[0] - o[0];
+  v[1] = e[1] + o1;
+  v[5] = e[1] - o1;
+  v[2] = e[2] + o2;
+  v[6] = e[2] - o2;
+  v[3] = e[3] + o3;
+  v[7] = e[3] - o3;
+}
+
+template <typename T>
+__device__ __forceinline__ void dft16(C2<T> *v, T dir) {
+  C2<T> e[8], o[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    e[k] = v[2 * k];
+    o[k] = v[2 * k + 1];
+  }
+  dft8(e, dir);
+  dft8(o, dir);
+  const T c1 = T(0.923879532511286756128183189396788933);
+  const T s1 = T(0.382683432365089771728459984030398866);
+  const T r = T(0.707106781186547524400844362104849039);
+  const T cs[8] = {T(1), c1, r, s1, T(0), -s1, -r, -c1};
+  const T sn[8] = {T(0), s1, r, c1, T(1), c1, r, s1};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const C2<T> w = {cs[k], dir * sn[k]};
+    const C2<T> t = cmul(o[k], w);
+    v[k] = e[k] + t;
+    v[k + 8] = e[k] - t;
+  }
+}
+
+// Odd prime P through symmetric pairs; g[q] = (cos 2 pi q/P, sin 2 pi q/P).
+template <int P, typename T>
+__device__ __forceinline__ void dftp(C2<T> *v, T dir, const C2<T> *__restrict__ g) {
+  constexpr int H = (P - 1) / 2;
+  C2<T> t[H], d[H];
+  const C2<T> x0 = v[0];
+  C2<T> sum = v[0];
+#pragma unroll
+  for (int m = 1; m <= H; ++m) {
+    t[m - 1] = v[m] + v[P - m];
+    d[m - 1] = v[m] - v[P - m];
+    sum = sum + t[m - 1];
+  }
+  C2<T> out[P];
+  out[0] = sum;
+#pragma unroll
+  for (int k = 1; k <= H; ++k) {
+    C2<T> a = x0, b = {T(0), T(0)};
+#pragma unroll
+    for (int m = 1; m <= H; ++m) {
+      const C2<T> w = g[(m * k) % P];
+      a = a + w.x * t[m - 1];
+      b = b + w.y * d[m - 1];
+    }
+    const C2<T> rb = rot(b, dir);
+    out[k] = a + rb;
+    out[P - k] = a - rb;
+  }
+#pragma unroll
+  for (int k = 0; k < P; ++k) v[k] = out[k];
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void dft(C2<T> *v, T dir, const C2<T> *g) {
+  if constexpr (R == 2) dft2(v);
+  else if constexpr (R == 3) dft3(v, dir);
+  else if constexpr (R == 4) dft4(v, dir);
+  else if constexpr (R == 5) dft5(v, dir);
+  else if constexpr (R == 8) dft8(v, dir);
+  else if constexpr (R == 16) dft16(v, dir);
+  else dftp<R>(v, dir, g);
+}
+
+// ------------------------------------------------------------------ stages
+// Stockham stage (radix R, ns = product of earlier radices): butterfly j of
+// N/R reads src[j + r N/R], multiplies by W_{ns R}^{(j mod ns) r}, and writes
+// dst[(j - k) R + k + r ns] with k = j mod ns.  Output is in natural order
+// after the last stage.
+
+template <int R, typename T>
+__device__ __forceinline__ void stage_fixed(const C2<T> *__restrict__ src,
+                                            C2<T> *__restrict__ dst, int L, int ld, int N,
+                                            int ns, const C2<T> *__restrict__ tw,
+                                            const C2<T> *__restrict__ g, T dir) {
+  const int NR = N / R;
+  const int total = L * NR;
+  const float invNR = 1.0f / float(NR), invns = 1.0f / float(ns);
+  for (int u = threadIdx.x; u < total; u += blockDim.x) {
+    const int l = fdiv(u, NR, invNR);
+    const int j = u - l * NR;
+    const int q = fdiv(j, ns, invns);
+    const int k = j - q * ns;
+    const C2<T> *s = src + l * ld + j;
+    C2<T> *d = dst + l * ld + q * ns * R + k;
+    C2<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = s[r * NR];
+    if (ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], dirw(tw[(r - 1) * ns + k], dir));
+    }
+    dft<R>(v, dir, g);
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[r * ns] = v[r];
+  }
+}
+
+// Any odd radix at run time: inputs are twiddled in place in src (each
+// butterfly owns its inputs), then every output pair is accumulated straight
+// from LDS.  O(R) work per element; used for primes above 13.
+template <typename T>
+__device__ void stage_generic(C2<T> *__restrict__ src, C2<T> *__restrict__ dst, int L,
+                              int ld, int N, int R, int ns, const C2<T> *__restrict__ tw,
+                              const C2<T> *__restrict__ g, T dir) {
+  const int NR = N / R;
+  const int total = L * NR;
+  const int H = (R - 1) / 2;
+  const float invNR = 1.0f / float(NR), invns = 1.0f / float(ns);
+  for (int u = threadIdx.x; u < total; u += blockDim.x) {
+    const int l = fdiv(u, NR, invNR);
+    const int j = u - l * NR;
+    const int q = fdiv(j, ns, invns);
+    const int k = j - q * ns;
+    C2<T> *s = src + l * ld + j;
+    C2<T> *d = dst + l * ld + q * ns * R + k;
+    if (ns > 1)
+      for (int r = 1; r < R; ++r) s[r * NR] = cmul(s[r * NR], dirw(tw[(r - 1) * ns + k], dir));
+    const C2<T> x0 = s[0];
+    C2<T> sum = x0;
+    for (int r = 1; r < R; ++r) sum = sum + s[r * NR];
+    d[0] = sum;
+    for (int kk = 1; kk <= H; ++kk) {
+      C2<T> a = x0, b = {T(0), T(0)};
+      int idx = 0;
+      for (int m = 1; m <= H; ++m) {
+        idx += kk;
+        if (idx >= R) idx -= R;
+        const C2<T> vm = s[m * NR], vp = s[(R - m) * NR];
+        const C2<T> w = g[idx];
+        a = a + w.x * (vm + vp);
+        b = b + w.y * (vm - vp);
+      }
+      const C2<T> rb = rot(b, dir);
+      d[kk * ns] = a + rb;
+      d[(R - kk) * ns] = a - rb;
+    }
+  }
+}
+
+// Runs all stages; returns the buffer holding the natural-order result.
+template <typename T>
+__device__ C2<T> *run_fft(const FftDesc &f, C2<T> *a, C2<T> *b, int L, int ld,
+                          const C2<T> *__restrict__ tw, const C2<T> *__restrict__ gen, T dir) {
+  for (int s = 0; s < f.nst; ++s) {
+    const Stage st = f.st[s];
+    const C2<T> *t = tw + st.twoff;
+    const C2<T> *g = gen + st.genoff;
+    switch (st.radix) {
+      case 2: stage_fixed<2>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 3: stage_fixed<3>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 4: stage_fixed<4>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 5: stage_fixed<5>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 7: stage_fixed<7>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 8: stage_fixed<8>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 11: stage_fixed<11>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 13: stage_fixed<13>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 16: stage_fixed<16>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      default: stage_generic(a, b, L, ld, f.n, st.radix, st.ns, t, g, dir); break;
+    }
+    __syncthreads();
+    C2<T> *x = a;
+    a = b;
+    b = x;
+  }
+  return a;
+}
+
+// ----------------------------------------------------------- line kernel
+
+enum LineOp {
+  OP_C2C = 0,     // plain complex lines
+  OP_PASSA = 1,   // four-step pass A (columns), inter-pass twiddle on store
+  OP_PASSB = 2,   // four-step pass B (rows)
+  OP_BLUE = 3,    // Bluestein entirely in LDS
+  OP_R2C_EVEN = 4,
+  OP_R2C_ODD = 5,
+  OP_C2R_EVEN = 6,
+  OP_C2R_ODD = 7,
+};
+enum SubFlags {
+  SUB_CHIRP = 1,   // pass A: multiply loads by the chirp, zero beyond n
+  SUB_KERNEL = 2,  // pass B: multiply by the Bluestein kernel (natural order)
+  SUB_FINAL = 4,   // pass B: multiply by the chirp and fct, keep m < n
+  SUB_CONJ = 8,    // Bluestein backward / r2c conj output / c2r conj input
+};
+
+template <typename T>
+struct LineArgs {
+  int64_t nlines;
+  int nb;
+  int64_t bdim[kMaxBatchDims], bsi[kMaxBatchDims], bso[kMaxBatchDims];
+  int64_t esi, eso;
+  const T *in;
+  T *out;
+  int L, log2L, ld, nin, nout, lfast_in, lfast_out;
+  FftDesc f;
+  const C2<T> *tw, *gen;
+  T dir, fct;
+  int sub;
+  int64_t bign;  // n of the transform (Bluestein / real), N of a four-step
+  int64_t other; // four-step: N2 in pass A, N1 in pass B
+  int shift;
+  const C2<T> *t0, *t1, *t2;  // op tables
+};
+
+template <bool V, typename T>
+__device__ __forceinline__ C2<T> ldc(const T *p) {
+  if constexpr (V) return *reinterpret_cast<const C2<T> *>(p);
+  else return C2<T>{p[0], p[1]};
+}
+template <bool V, typename T>
+__device__ __forceinline__ void stc(T *p, C2<T> v) {
+  if constexpr (V) *reinterpret_cast<C2<T> *>(p) = v;
+  else {
+    p[0] = v.x;
+    p[1] = v.y;
+  }
+}
+
+// visit (l, j) for l < nl, j < cnt; l-fastest when lines interleave in memory
+template <typename F>
+__device__ __forceinline__ void for_tile(int nl, int cnt, int lfast, int log2L, F &&f) {
+  if (lfast) {
+    const int Lm = (1 << log2L) - 1;
+    const int total = cnt << log2L;
+    for (int u = threadIdx.x; u < total; u += blockDim.x) {
+      const int l = u & Lm, j = u >> log2L;
+      if (l < nl) f(l, j);
+    }
+  } else {
+    const int total = nl * cnt;
+    const float inv = 1.0f / float(cnt);
+    for (int u = threadIdx.x; u < total; u += blockDim.x) {
+      const int l = fdiv(u, cnt, inv);
+      f(l, u - l * cnt);
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ C2<T> big_twiddle(const LineArgs<T> &a, int64_t e) {
+  const int64_t mask = (int64_t(1) << a.shift) - 1;
+  return cmul(a.t0[e & mask], a.t1[e >> a.shift]);
+}
+
+}  // namespace dev
+
+// Compile-time specialised kernels (fast.hip); false when none matches.
+template <typename T>
+bool launch_fast(int op, const dev::LineArgs<T> &a, bool lf_in, bool lf_out, bool vin,
+                 bool vout, hipStream_t s);
+extern bool g_fast_enabled;
+
+}  // namespace rfh

@@ -1,0 +1,378 @@
+// fast.hip -- compile-time specialised line kernels for power-of-two lines.
+//
+// Same operations as k_lines (kernels.hip) but with the line length, radix
+// schedule and thread mapping fixed at compile time:
+//   * stage 0 reads its radix-R inputs straight from HBM into registers and
+//     the last stage writes straight back, so a line crosses LDS only
+//     between stages (S-1 exchanges instead of S+1);
+//   * one LDS buffer per line, padded by one element every 16 (complex64) or
+//     8 (complex128) elements and one per line, instead of two ping-pong
+//     buffers -- twice the lines per CU;
+//   * threads are mapped to (line, butterfly) so that global accesses are
+//     coalesced: consecutive lanes take consecutive elements of one line
+//     (contiguous lines), or the same element of adjacent lines
+//     (interleaved lines, LFAST).  When input and output want different
+//     mappings, or an operation needs two bins of a line at once (r2c), the
+//     result goes through LDS and a store loop with the output mapping.
+// Twiddles come from the same per-stage tables as k_lines (the plan's
+// Direct for this length uses the same radix schedule).
+#include <mutex>
+#include <set>
+
+#include "device.hpp"
+
+namespace rfh {
+namespace {
+using namespace dev;
+
+template <int... Rs>
+struct Sched {
+  static constexpr int count = sizeof...(Rs);
+  static constexpr int at(int s) {
+    constexpr int a[] = {Rs...};
+    return a[s];
+  }
+  static constexpr int ns(int s) {
+    int p = 1;
+    for (int i = 0; i < s; ++i) p *= at(i);
+    return p;
+  }
+};
+
+template <typename T>
+struct PadOf {
+  static constexpr int shift = sizeof(T) == 4 ? 4 : 3;
+};
+
+template <typename T>
+__device__ __forceinline__ int padi(int i) {
+  return i + (i >> PadOf<T>::shift);
+}
+
+template <typename T, int N>
+constexpr int pitch_of() {
+  return N + (N >> PadOf<T>::shift) + 1;
+}
+
+// per-thread state of one fast-kernel invocation
+template <typename T>
+struct FCtx {
+  const LineArgs<T> *a;
+  C2<T> *line;     // this thread's line in LDS
+  const T *gin;    // this thread's line in global memory (input)
+  T *gout;         // ... output
+  int aux;
+  bool valid;
+  int i;           // index of the thread within its line
+};
+
+template <typename T, int OP, bool VIN>
+__device__ __forceinline__ C2<T> fload(const FCtx<T> &c, int p) {
+  const LineArgs<T> &a = *c.a;
+  if constexpr (OP == OP_R2C_EVEN) {
+    if constexpr (VIN)
+      return ldc<true>(c.gin + 2 * p);
+    else
+      return C2<T>{c.gin[2 * p * a.esi], c.gin[(2 * p + 1) * a.esi]};
+  } else if constexpr (OP == OP_C2R_EVEN) {
+    const int h = a.f.n;
+    C2<T> xj = ldc<VIN>(c.gin + p * a.esi), xm = ldc<VIN>(c.gin + (h - p) * a.esi);
+    if (a.sub & SUB_CONJ) {
+      xj.y = -xj.y;
+      xm.y = -xm.y;
+    }
+    if (p == 0) {
+      xj.y = T(0);
+      xm.y = T(0);
+    }
+    const C2<T> A = xj + conjc(xm);
+    const C2<T> B = cmul(xj - conjc(xm), conjc(a.t0[p]));
+    return C2<T>{A.x - B.y, A.y + B.x};
+  } else {
+    return ldc<VIN>(c.gin + p * a.esi);
+  }
+}
+
+// elementwise store of bin p of this thread's line (all ops except r2c)
+template <typename T, int OP, bool VOUT>
+__device__ __forceinline__ void fstore(const LineArgs<T> &a, T *gout, int aux, int p, C2<T> v) {
+  if constexpr (OP == OP_C2C) {
+    stc<VOUT>(gout + p * a.eso, a.fct * v);
+  } else if constexpr (OP == OP_PASSA) {
+    const C2<T> w = dirw(big_twiddle(a, int64_t(p) * aux), a.dir);
+    stc<VOUT>(gout + p * a.eso, cmul(v, w));
+  } else if constexpr (OP == OP_PASSB) {
+    const int64_t m = aux + int64_t(p) * a.other;
+    const bool cj = (a.sub & SUB_CONJ) != 0;
+    if (a.sub & SUB_KERNEL) {
+      const C2<T> w = a.t1[m];
+      stc<VOUT>(gout + p * a.eso, cmul(v, cj ? conjc(w) : w));
+    } else if (a.sub & SUB_FINAL) {
+      if (m < a.bign) {
+        const C2<T> ch = a.t0[m];
+        stc<VOUT>(gout + p * a.eso, a.fct * cmul(v, cj ? ch : conjc(ch)));
+      }
+    } else {
+      stc<VOUT>(gout + p * a.eso, a.fct * v);
+    }
+  } else if constexpr (OP == OP_C2R_EVEN) {
+    const C2<T> z = a.fct * v;
+    if constexpr (VOUT) {
+      stc<true>(gout + 2 * p, z);
+    } else {
+      gout[2 * p * a.eso] = z.x;
+      gout[(2 * p + 1) * a.eso] = z.y;
+    }
+  }
+}
+
+// r2c post-processing of bin k (0..h) from the natural-order half-length
+// spectrum in LDS
+template <typename T>
+__device__ __forceinline__ C2<T> r2c_bin(const LineArgs<T> &a, const C2<T> *line, int k) {
+  const int h = a.f.n;
+  C2<T> X;
+  if (k == 0 || k == h) {
+    const C2<T> z = line[0];
+    X = C2<T>{k == 0 ? z.x + z.y : z.x - z.y, T(0)};
+  } else {
+    const C2<T> zk = line[padi<T>(k)], zm = conjc(line[padi<T>(h - k)]);
+    const C2<T> fe = T(0.5) * (zk + zm);
+    const C2<T> dd = zk - zm;
+    const C2<T> fo = C2<T>{T(0.5) * dd.y, T(-0.5) * dd.x};
+    X = fe + cmul(a.t0[k], fo);
+  }
+  if (a.sub & SUB_CONJ) X.y = -X.y;
+  return a.fct * X;
+}
+
+template <typename T, int N, int TPL, int OP, bool VIN, bool VOUT, bool DIRECT_OUT,
+          typename SC, int S>
+__device__ __forceinline__ void fstage(const FCtx<T> &c) {
+  constexpr int R = SC::at(S);
+  constexpr int NS = SC::ns(S);
+  constexpr int NR = N / R;
+  constexpr int Q = (NR + TPL - 1) / TPL;
+  constexpr bool FIRST = S == 0;
+  constexpr bool LAST = S == SC::count - 1;
+  const LineArgs<T> &a = *c.a;
+  const T dir = a.dir;
+  C2<T> v[Q][R];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = c.i + q * TPL;
+    if (Q * TPL == NR || j < NR) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if constexpr (FIRST)
+          v[q][r] = c.valid ? fload<T, OP, VIN>(c, j + r * NR) : C2<T>{T(0), T(0)};
+        else
+          v[q][r] = c.line[padi<T>(j + r * NR)];
+      }
+      if constexpr (NS > 1) {
+        const int k = j % NS;
+        const C2<T> *tw = a.tw + a.f.st[S].twoff;
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], dirw(tw[(r - 1) * NS + k], dir));
+      }
+      dft<R>(v[q], dir, a.gen);
+    }
+  }
+  if constexpr (LAST && DIRECT_OUT) {
+    if (c.valid) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int j = c.i + q * TPL;
+        if (Q * TPL == NR || j < NR) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) fstore<T, OP, VOUT>(a, c.gout, c.aux, j + r * NS, v[q][r]);
+        }
+      }
+    }
+    return;
+  } else {
+    if constexpr (!FIRST) __syncthreads();  // every read of this stage is done
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = c.i + q * TPL;
+      if (Q * TPL == NR || j < NR) {
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 0; r < R; ++r) c.line[padi<T>((j - k) * R + k + r * NS)] = v[q][r];
+      }
+    }
+    __syncthreads();
+    if constexpr (!LAST)
+      fstage<T, N, TPL, OP, VIN, VOUT, DIRECT_OUT, SC, S + 1>(c);
+  }
+}
+
+template <typename T, int N, int TPL, int LPB, bool LF_IN, bool LF_OUT, int OP, bool VIN,
+          bool VOUT, typename SC>
+__global__ void __launch_bounds__(TPL *LPB) k_fast(const LineArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int P = pitch_of<T, N>();
+  constexpr bool DIRECT_OUT = (LF_IN == LF_OUT) && OP != OP_R2C_EVEN;
+  C2<T> *lds = reinterpret_cast<C2<T> *>(smem_raw);
+  __shared__ int64_t s_oo[LPB];
+  __shared__ int s_aux[LPB];
+  const int t = threadIdx.x;
+  const int l = LF_IN ? t % LPB : t / TPL;
+  const int i = LF_IN ? t / LPB : t % TPL;
+  const int64_t b = int64_t(blockIdx.x) * LPB + l;
+  FCtx<T> c;
+  c.a = &a;
+  c.line = lds + l * P;
+  c.valid = b < a.nlines;
+  c.i = i;
+  int64_t oi = 0, oo = 0;
+  int aux = 0;
+  if (c.valid) {
+    int64_t rem = b;
+    for (int d = a.nb - 1; d >= 0; --d) {
+      const int64_t q = rem / a.bdim[d];
+      const int64_t idx = rem - q * a.bdim[d];
+      rem = q;
+      oi += idx * a.bsi[d];
+      oo += idx * a.bso[d];
+      if (d == a.nb - 1) aux = int(idx);
+    }
+  }
+  c.gin = a.in + oi;
+  c.gout = a.out + oo;
+  c.aux = aux;
+  if constexpr (!DIRECT_OUT) {
+    if (i == 0) {
+      s_oo[l] = oo;
+      s_aux[l] = aux;
+    }
+  }
+  fstage<T, N, TPL, OP, VIN, VOUT, DIRECT_OUT, SC, 0>(c);
+  if constexpr (!DIRECT_OUT) {
+    // natural-order result of every line is in LDS; store with the output
+    // mapping (the last stage ended with a barrier)
+    const int nl = int(min(int64_t(LPB), a.nlines - int64_t(blockIdx.x) * LPB));
+    const int lo = LF_OUT ? t % LPB : t / TPL;
+    const int io = LF_OUT ? t / LPB : t % TPL;
+    if (lo < nl) {
+      const C2<T> *ln = lds + lo * P;
+      T *g = a.out + s_oo[lo];
+      if constexpr (OP == OP_R2C_EVEN) {
+        for (int k = io; k <= N; k += TPL) stc<VOUT>(g + k * a.eso, r2c_bin(a, ln, k));
+      } else {
+        for (int k = io; k < N; k += TPL)
+          fstore<T, OP, VOUT>(a, g, s_aux[lo], k, ln[padi<T>(k)]);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ dispatch
+
+std::mutex g_fast_mu;
+std::set<const void *> g_fast_attr;
+
+template <typename T, int N, int TPL, int LPB, bool LF_IN, bool LF_OUT, int OP, bool VIN,
+          bool VOUT, typename SC>
+void go(const LineArgs<T> &a, hipStream_t s) {
+  const void *fn = reinterpret_cast<const void *>(
+      &k_fast<T, N, TPL, LPB, LF_IN, LF_OUT, OP, VIN, VOUT, SC>);
+  const size_t lds = size_t(LPB) * pitch_of<T, N>() * sizeof(C2<T>);
+  if (lds > 65536) {
+    std::lock_guard<std::mutex> lk(g_fast_mu);
+    if (!g_fast_attr.count(fn)) {
+      hipFuncAttributes fa;
+      HIP_CHECK(hipFuncGetAttributes(&fa, fn));
+      HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(lds_limit() - fa.sharedSizeBytes)));
+      g_fast_attr.insert(fn);
+    }
+  }
+  const int64_t groups = (a.nlines + LPB - 1) / LPB;
+  hipLaunchKernelGGL((k_fast<T, N, TPL, LPB, LF_IN, LF_OUT, OP, VIN, VOUT, SC>),
+                     dim3(unsigned(groups)), dim3(TPL * LPB), lds, s, a);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    RFH_THROW(2, std::string("fast line kernel launch: ") + hipGetErrorString(e));
+}
+
+template <typename T, int N, int TPL, int LPB, int OP, typename SC>
+bool go_map(const LineArgs<T> &a, bool lf_in, bool lf_out, bool vin, bool vout,
+            hipStream_t s) {
+#define RFH_GO(LI, LO, VI, VO)                                    \
+  if (lf_in == LI && lf_out == LO && vin == VI && vout == VO) {   \
+    go<T, N, TPL, LPB, LI, LO, OP, VI, VO, SC>(a, s);             \
+    return true;                                                  \
+  }
+  RFH_GO(false, false, true, true)
+  RFH_GO(true, true, true, true)
+  RFH_GO(false, true, true, true)
+  RFH_GO(true, false, true, true)
+#undef RFH_GO
+  return false;  // unaligned layouts use the generic kernel
+}
+
+// threads per line and lines per block for length N
+template <typename T, int N, int OP, typename SC>
+bool go_len(const LineArgs<T> &a, bool lf_in, bool lf_out, bool vin, bool vout,
+            hipStream_t s) {
+  constexpr int E = 16;  // elements per thread (largest radix)
+  constexpr int TPL = N >= E ? N / E : 1;
+  // contiguous lines: 256-thread blocks (one line per block from N = 4096)
+  constexpr int LPB_C = TPL >= 256 ? 1 : 256 / TPL;
+  // interleaved lines: at least 8 adjacent lines per block (64-byte rows)
+  constexpr int LPB_I = LPB_C >= 8 ? LPB_C : 8;
+  if (lf_in || lf_out) {
+    if constexpr (TPL * LPB_I <= 1024 &&
+                  size_t(LPB_I) * pitch_of<T, N>() * sizeof(C2<T>) <= 152 * 1024)
+      return go_map<T, N, TPL, LPB_I, OP, SC>(a, lf_in, lf_out, vin, vout, s);
+    return false;
+  }
+  return go_map<T, N, TPL, LPB_C, OP, SC>(a, lf_in, lf_out, vin, vout, s);
+}
+
+template <typename T, int OP>
+bool go_op(const LineArgs<T> &a, bool lf_in, bool lf_out, bool vin, bool vout,
+           hipStream_t s) {
+  switch (a.f.n) {
+    case 2: return go_len<T, 2, OP, Sched<2>>(a, lf_in, lf_out, vin, vout, s);
+    case 4: return go_len<T, 4, OP, Sched<4>>(a, lf_in, lf_out, vin, vout, s);
+    case 8: return go_len<T, 8, OP, Sched<8>>(a, lf_in, lf_out, vin, vout, s);
+    case 16: return go_len<T, 16, OP, Sched<16>>(a, lf_in, lf_out, vin, vout, s);
+    case 32: return go_len<T, 32, OP, Sched<16, 2>>(a, lf_in, lf_out, vin, vout, s);
+    case 64: return go_len<T, 64, OP, Sched<16, 4>>(a, lf_in, lf_out, vin, vout, s);
+    case 128: return go_len<T, 128, OP, Sched<16, 8>>(a, lf_in, lf_out, vin, vout, s);
+    case 256: return go_len<T, 256, OP, Sched<16, 16>>(a, lf_in, lf_out, vin, vout, s);
+    case 512: return go_len<T, 512, OP, Sched<16, 16, 2>>(a, lf_in, lf_out, vin, vout, s);
+    case 1024: return go_len<T, 1024, OP, Sched<16, 16, 4>>(a, lf_in, lf_out, vin, vout, s);
+    case 2048: return go_len<T, 2048, OP, Sched<16, 16, 8>>(a, lf_in, lf_out, vin, vout, s);
+    case 4096: return go_len<T, 4096, OP, Sched<16, 16, 16>>(a, lf_in, lf_out, vin, vout, s);
+    default: return false;
+  }
+}
+
+}  // namespace
+
+bool g_fast_enabled = true;
+
+// Launches the specialised kernel for (op, length, layout) when one exists.
+template <typename T>
+bool launch_fast(int op, const LineArgs<T> &a, bool lf_in, bool lf_out, bool vin, bool vout,
+                 hipStream_t s) {
+  if (!g_fast_enabled) return false;
+  switch (op) {
+    case OP_C2C: return go_op<T, OP_C2C>(a, lf_in, lf_out, vin, vout, s);
+    case OP_PASSA: return go_op<T, OP_PASSA>(a, lf_in, lf_out, vin, vout, s);
+    case OP_PASSB: return go_op<T, OP_PASSB>(a, lf_in, lf_out, vin, vout, s);
+    case OP_R2C_EVEN: return go_op<T, OP_R2C_EVEN>(a, lf_in, lf_out, vin, vout, s);
+    case OP_C2R_EVEN: return go_op<T, OP_C2R_EVEN>(a, lf_in, lf_out, vin, vout, s);
+    default: return false;
+  }
+}
+
+template bool launch_fast<float>(int, const dev::LineArgs<float> &, bool, bool, bool, bool,
+                                 hipStream_t);
+template bool launch_fast<double>(int, const dev::LineArgs<double> &, bool, bool, bool, bool,
+                                  hipStream_t);
+
+}  // namespace rfh

@@ -1,0 +1,920 @@
+// kernels.hip -- gfx950 kernels of the transform engine.
+//
+// The workhorse is k_lines: one workgroup owns L whole lines, loads them
+// (coalesced: consecutive lanes take consecutive elements of a line, or the
+// same element of adjacent lines when the lines are interleaved in memory),
+// runs every Stockham stage of the line FFT out of LDS (two ping-pong
+// buffers, one barrier per stage), and stores.  Per-operation load/store
+// hooks fuse the reference's pre/post processing into the same pass:
+//   * c2c with the output factor fct                (ExecC2C, H:3646-3659)
+//   * real -> n/2+1 bins and back through a half-length complex FFT
+//     (general_r2c / general_c2r, H:3723-3852)
+//   * Bluestein chirp, kernel product and second FFT (fftblue, H:2723-2762)
+//   * the two passes of a four-step transform for lines longer than LDS,
+//     with the inter-pass twiddle and the Bluestein steps folded in.
+// (H = rocket_fft/_pocketfft_hdronly.h.)  Remaining real transforms (DCT/DST,
+// Hartley, FFTPACK halfcomplex and real lengths without a fused plan) use
+// k_pre -> complex lines -> k_post around the same c2c machinery.
+#include <cmath>
+#include <mutex>
+#include <set>
+
+#include "device.hpp"
+
+namespace rfh {
+namespace {
+using namespace dev;
+
+template <typename T, int OP, bool VIN, bool VOUT>
+__global__ void __launch_bounds__(kThreads) k_lines(const LineArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ int64_t s_oi[kMaxLinesPerGroup], s_oo[kMaxLinesPerGroup];
+  __shared__ int s_aux[kMaxLinesPerGroup];
+  C2<T> *bufA = reinterpret_cast<C2<T> *>(smem_raw);
+  C2<T> *bufB = bufA + a.L * a.ld;
+  const int64_t b0 = int64_t(blockIdx.x) * a.L;
+  const int nl = int(min(int64_t(a.L), a.nlines - b0));
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+    int64_t rem = b0 + l, oi = 0, oo = 0;
+    int aux = 0;
+    for (int d = a.nb - 1; d >= 0; --d) {
+      const int64_t q = rem / a.bdim[d];
+      const int64_t idx = rem - q * a.bdim[d];
+      rem = q;
+      oi += idx * a.bsi[d];
+      oo += idx * a.bso[d];
+      if (d == a.nb - 1) aux = int(idx);
+    }
+    s_oi[l] = oi;
+    s_oo[l] = oo;
+    s_aux[l] = aux;
+  }
+  __syncthreads();
+
+  const int ld = a.ld;
+  const bool conjf = (a.sub & SUB_CONJ) != 0;
+  // ---------------------------------------------------------------- load
+  for_tile(nl, a.nin, a.lfast_in, a.log2L, [&](int l, int j) {
+    const T *base = a.in + s_oi[l];
+    C2<T> v;
+    if constexpr (OP == OP_C2C || OP == OP_PASSB) {
+      v = ldc<VIN>(base + j * a.esi);
+    } else if constexpr (OP == OP_PASSA) {
+      if (a.sub & SUB_CHIRP) {
+        const int64_t m = int64_t(j) * a.other + s_aux[l];
+        if (m < a.bign) {
+          const C2<T> c = a.t2[m];
+          v = cmul(ldc<VIN>(base + j * a.esi), conjf ? c : conjc(c));
+        } else {
+          v = C2<T>{T(0), T(0)};
+        }
+      } else {
+        v = ldc<VIN>(base + j * a.esi);
+      }
+    } else if constexpr (OP == OP_BLUE) {
+      if (j < a.bign) {
+        const C2<T> c = a.t0[j];
+        v = cmul(ldc<VIN>(base + j * a.esi), conjf ? c : conjc(c));
+      } else {
+        v = C2<T>{T(0), T(0)};
+      }
+    } else if constexpr (OP == OP_R2C_EVEN) {
+      if constexpr (VIN)  // contiguous real line: one 2-wide load
+        v = ldc<true>(base + 2 * j);
+      else
+        v = C2<T>{base[2 * j * a.esi], base[(2 * j + 1) * a.esi]};
+    } else if constexpr (OP == OP_R2C_ODD) {
+      v = C2<T>{base[j * a.esi], T(0)};
+    } else if constexpr (OP == OP_C2R_EVEN) {
+      const int h = a.f.n;
+      C2<T> xj = ldc<VIN>(base + j * a.esi), xm = ldc<VIN>(base + (h - j) * a.esi);
+      if (conjf) {
+        xj.y = -xj.y;
+        xm.y = -xm.y;
+      }
+      if (j == 0) {
+        xj.y = T(0);
+        xm.y = T(0);
+      }
+      const C2<T> A = xj + conjc(xm);
+      const C2<T> B = cmul(xj - conjc(xm), conjc(a.t0[j]));
+      v = C2<T>{A.x - B.y, A.y + B.x};
+    } else {  // OP_C2R_ODD
+      const int n = a.f.n;
+      if (j == 0) {
+        v = C2<T>{base[0], T(0)};
+      } else if (2 * j < n) {
+        v = ldc<VIN>(base + j * a.esi);
+        if (conjf) v.y = -v.y;
+      } else {
+        v = ldc<VIN>(base + (n - j) * a.esi);
+        if (!conjf) v.y = -v.y;
+      }
+    }
+    bufA[l * ld + j] = v;
+  });
+  __syncthreads();
+
+  // ----------------------------------------------------------------- FFT
+  C2<T> *res = run_fft<T>(a.f, bufA, bufB, a.L, ld, a.tw, a.gen, a.dir);
+  if constexpr (OP == OP_BLUE) {
+    const int n2 = a.f.n;
+    for_tile(a.L, n2, 0, a.log2L, [&](int l, int k) {
+      const C2<T> w = a.t1[k];
+      res[l * ld + k] = cmul(res[l * ld + k], conjf ? conjc(w) : w);
+    });
+    __syncthreads();
+    res = run_fft<T>(a.f, res, res == bufA ? bufB : bufA, a.L, ld, a.tw, a.gen, -a.dir);
+  }
+
+  // --------------------------------------------------------------- store
+  for_tile(nl, a.nout, a.lfast_out, a.log2L, [&](int l, int k) {
+    T *base = a.out + s_oo[l];
+    const C2<T> *r = res + l * ld;
+    if constexpr (OP == OP_C2C) {
+      stc<VOUT>(base + k * a.eso, a.fct * r[k]);
+    } else if constexpr (OP == OP_PASSA) {
+      const C2<T> w = dirw(big_twiddle(a, int64_t(k) * s_aux[l]), a.dir);
+      stc<VOUT>(base + k * a.eso, cmul(r[k], w));
+    } else if constexpr (OP == OP_PASSB) {
+      const int64_t m = s_aux[l] + int64_t(k) * a.other;
+      if (a.sub & SUB_KERNEL) {
+        const C2<T> w = a.t1[m];
+        stc<VOUT>(base + k * a.eso, cmul(r[k], conjf ? conjc(w) : w));
+      } else if (a.sub & SUB_FINAL) {
+        if (m < a.bign) {
+          const C2<T> c = a.t0[m];
+          stc<VOUT>(base + k * a.eso, a.fct * cmul(r[k], conjf ? c : conjc(c)));
+        }
+      } else {
+        stc<VOUT>(base + k * a.eso, a.fct * r[k]);
+      }
+    } else if constexpr (OP == OP_BLUE) {
+      const C2<T> c = a.t0[k];
+      stc<VOUT>(base + k * a.eso, a.fct * cmul(r[k], conjf ? c : conjc(c)));
+    } else if constexpr (OP == OP_R2C_EVEN) {
+      const int h = a.f.n;
+      C2<T> X;
+      if (k == 0 || k == h) {
+        const C2<T> z = r[0];
+        X = C2<T>{k == 0 ? z.x + z.y : z.x - z.y, T(0)};
+      } else {
+        const C2<T> zk = r[k], zm = conjc(r[h - k]);
+        const C2<T> fe = T(0.5) * (zk + zm);
+        const C2<T> dd = zk - zm;
+        const C2<T> fo = C2<T>{T(0.5) * dd.y, T(-0.5) * dd.x};  // (zk - zm) / 2i
+        X = fe + cmul(a.t0[k], fo);
+      }
+      if (conjf) X.y = -X.y;
+      stc<VOUT>(base + k * a.eso, a.fct * X);
+    } else if constexpr (OP == OP_R2C_ODD) {
+      C2<T> X = r[k];
+      if (conjf) X.y = -X.y;
+      stc<VOUT>(base + k * a.eso, a.fct * X);
+    } else if constexpr (OP == OP_C2R_EVEN) {
+      const C2<T> z = a.fct * r[k];
+      if constexpr (VOUT) {  // contiguous real line: one 2-wide store
+        stc<true>(base + 2 * k, z);
+      } else {
+        base[2 * k * a.eso] = z.x;
+        base[(2 * k + 1) * a.eso] = z.y;
+      }
+    } else {  // OP_C2R_ODD
+      base[k * a.eso] = a.fct * r[k].x;
+    }
+  });
+}
+
+// ----------------------------------------------------------- host launch
+
+int pow2floor(int64_t v) {
+  int p = 1;
+  while (int64_t(p) * 2 <= v) p *= 2;
+  return p;
+}
+
+int ilog2(int v) {
+  int l = 0;
+  while ((1 << l) < v) ++l;
+  return l;
+}
+
+// lines per group: the largest power of two whose two LDS buffers fit
+// `budget` (and are not wasted on missing lines)
+int choose_lines(int ld, size_t elem, int64_t nlines, bool interleaved) {
+  const size_t budget = std::min<size_t>(interleaved ? 96 * 1024 : 48 * 1024,
+                                         lds_limit() - 8192);
+  int L = 1;
+  while (L * 2 <= kMaxLinesPerGroup && size_t(2) * (L * 2) * ld * elem <= budget &&
+         int64_t(L) < nlines)
+    L *= 2;
+  return L;
+}
+
+bool lfast_for(int64_t es, const LineSet &g, bool input) {
+  if (g.nb == 0) return false;
+  const int64_t bs = input ? g.bsi[g.nb - 1] : g.bso[g.nb - 1];
+  return std::llabs(es) > std::llabs(bs);
+}
+
+// complex accesses may use 2-wide loads when base and every stride are even
+bool pairs_aligned(const void *p, size_t scalar, int64_t es, const int64_t *bs, int nb) {
+  if (reinterpret_cast<uintptr_t>(p) % (2 * scalar)) return false;
+  if (es % 2) return false;
+  for (int d = 0; d < nb; ++d)
+    if (bs[d] % 2) return false;
+  return true;
+}
+
+std::mutex g_attr_mu;
+std::set<const void *> g_attr_done;
+
+// Raises a kernel's dynamic-LDS ceiling to what the device allows beside the
+// kernel's static LDS (once per kernel).
+void allow_lds(const void *fn, size_t bytes) {
+  if (bytes <= 65536) return;
+  std::lock_guard<std::mutex> lk(g_attr_mu);
+  if (g_attr_done.count(fn)) return;
+  hipFuncAttributes fa;
+  HIP_CHECK(hipFuncGetAttributes(&fa, fn));
+  const int dyn = int(lds_limit()) - int(fa.sharedSizeBytes);
+  if (int64_t(bytes) > dyn)
+    RFH_THROW(3, "line kernel needs " + std::to_string(bytes) + " B of LDS beside " +
+                     std::to_string(fa.sharedSizeBytes) + " B static; device allows " +
+                     std::to_string(lds_limit()));
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+  if (e != hipSuccess)
+    RFH_THROW(2, std::string("hipFuncSetAttribute(MaxDynamicSharedMemorySize, ") +
+                     std::to_string(dyn) + "): " + hipGetErrorString(e));
+  g_attr_done.insert(fn);
+}
+
+template <typename T, int OP, bool VIN, bool VOUT>
+void launch_one(const LineArgs<T> &a, size_t lds, hipStream_t s) {
+  const void *fn = reinterpret_cast<const void *>(&k_lines<T, OP, VIN, VOUT>);
+  allow_lds(fn, lds);
+  const int64_t groups = (a.nlines + a.L - 1) / a.L;
+  if (groups > 0x7fffffff / kThreads) RFH_THROW(3, "batch too large for one launch");
+  hipLaunchKernelGGL((k_lines<T, OP, VIN, VOUT>), dim3(unsigned(groups)), dim3(kThreads), lds,
+                     s, a);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    RFH_THROW(2, std::string("line kernel launch (") + std::to_string(groups) + " groups, " +
+                     std::to_string(lds) + " B LDS, op " + std::to_string(OP) +
+                     "): " + hipGetErrorString(e));
+}
+
+template <typename T, int OP>
+void launch_var(const LineArgs<T> &a, size_t lds, bool vin, bool vout, hipStream_t s) {
+  if (vin && vout) launch_one<T, OP, true, true>(a, lds, s);
+  else if (vin) launch_one<T, OP, true, false>(a, lds, s);
+  else if (vout) launch_one<T, OP, false, true>(a, lds, s);
+  else launch_one<T, OP, false, false>(a, lds, s);
+}
+
+struct PassSpec {
+  const Direct *d;
+  int op;
+  int sub = 0;
+  int dir = -1;
+  double fct = 1.0;
+  int64_t bign = 0, other = 0;
+  int shift = 0;
+  const void *t0 = nullptr, *t1 = nullptr, *t2 = nullptr;
+  int nin = 0, nout = 0;
+  int lines = 0;        // 0 = choose
+  bool real_in = false, real_out = false;
+};
+
+template <typename T>
+void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
+  if (g.nlines <= 0) return;
+  LineArgs<T> a{};
+  a.nlines = g.nlines;
+  a.nb = g.nb;
+  for (int d = 0; d < g.nb; ++d) {
+    a.bdim[d] = g.bdim[d];
+    a.bsi[d] = g.bsi[d];
+    a.bso[d] = g.bso[d];
+  }
+  a.esi = g.esi;
+  a.eso = g.eso;
+  a.in = static_cast<const T *>(g.in);
+  a.out = static_cast<T *>(g.out);
+  a.f = ps.d->desc;
+  a.ld = a.f.n;
+  a.nin = ps.nin;
+  a.nout = ps.nout;
+  a.lfast_in = lfast_for(g.esi, g, true);
+  a.lfast_out = lfast_for(g.eso, g, false);
+  const size_t elem = 2 * sizeof(T);
+  a.L = ps.lines ? ps.lines
+                 : choose_lines(a.ld, elem, g.nlines, a.lfast_in || a.lfast_out);
+  a.log2L = ilog2(a.L);
+  a.tw = static_cast<const C2<T> *>(ps.d->tw.p);
+  a.gen = static_cast<const C2<T> *>(ps.d->gen.p);
+  a.dir = T(ps.dir);
+  a.fct = T(ps.fct);
+  a.sub = ps.sub;
+  a.bign = ps.bign;
+  a.other = ps.other;
+  a.shift = ps.shift;
+  a.t0 = static_cast<const C2<T> *>(ps.t0);
+  a.t1 = static_cast<const C2<T> *>(ps.t1);
+  a.t2 = static_cast<const C2<T> *>(ps.t2);
+  const size_t lds = size_t(2) * a.L * a.ld * elem;
+  if (lds + 8192 > lds_limit()) RFH_THROW(3, "line does not fit in LDS");
+  bool vin, vout;
+  if (ps.real_in)  // real pairs: 2-wide loads when the line is contiguous
+    vin = g.esi == 1 && pairs_aligned(g.in, sizeof(T), 0, g.bsi, g.nb);
+  else
+    vin = pairs_aligned(g.in, sizeof(T), g.esi, g.bsi, g.nb);
+  if (ps.real_out)
+    vout = g.eso == 1 && pairs_aligned(g.out, sizeof(T), 0, g.bso, g.nb);
+  else
+    vout = pairs_aligned(g.out, sizeof(T), g.eso, g.bso, g.nb);
+  if (launch_fast<T>(ps.op, a, a.lfast_in, a.lfast_out, vin, vout, s)) return;
+  switch (ps.op) {
+    case OP_C2C: launch_var<T, OP_C2C>(a, lds, vin, vout, s); break;
+    case OP_PASSA: launch_var<T, OP_PASSA>(a, lds, vin, vout, s); break;
+    case OP_PASSB: launch_var<T, OP_PASSB>(a, lds, vin, vout, s); break;
+    case OP_BLUE: launch_var<T, OP_BLUE>(a, lds, vin, vout, s); break;
+    case OP_R2C_EVEN: launch_var<T, OP_R2C_EVEN>(a, lds, vin, vout, s); break;
+    case OP_R2C_ODD: launch_var<T, OP_R2C_ODD>(a, lds, false, vout, s); break;
+    case OP_C2R_EVEN: launch_var<T, OP_C2R_EVEN>(a, lds, vin, vout, s); break;
+    case OP_C2R_ODD: launch_var<T, OP_C2R_ODD>(a, lds, vin, false, s); break;
+  }
+}
+
+void run_pass_any(bool dbl, const PassSpec &ps, const LineSet &g, hipStream_t s) {
+  if (dbl)
+    run_pass<double>(ps, g, s);
+  else
+    run_pass<float>(ps, g, s);
+}
+
+// Line sets of the two passes of a four-step transform of length N = n1*n2
+// over the lines of g.  Pass A: line (b, c) = column c of line b, elements
+// n1 apart in units of the column stride; pass B: row r of the n1 x n2
+// intermediate, stored to position r + n1*k of the output line.
+LineSet pass_a_lines(const LineSet &g, uint64_t n1, uint64_t n2, const void *in,
+                     int64_t esi_line, void *tmp) {
+  LineSet a{};
+  a.in = in;
+  a.out = tmp;
+  a.nlines = g.nlines * int64_t(n2);
+  a.nb = g.nb + 1;
+  if (a.nb > kMaxBatchDims) RFH_THROW(3, "too many batch dimensions");
+  int64_t tstride = int64_t(2 * n1 * n2);
+  for (int d = g.nb - 1; d >= 0; --d) {
+    a.bdim[d] = g.bdim[d];
+    a.bsi[d] = g.bsi[d];
+    a.bso[d] = tstride;
+    tstride *= g.bdim[d];
+  }
+  a.bdim[g.nb] = int64_t(n2);
+  a.bsi[g.nb] = esi_line;
+  a.bso[g.nb] = 2;
+  a.esi = esi_line * int64_t(n2);
+  a.eso = 2 * int64_t(n2);
+  return a;
+}
+
+LineSet pass_b_lines(const LineSet &g, uint64_t n1, uint64_t n2, const void *tmp, void *out,
+                     int64_t eso_line, const int64_t *bso) {
+  LineSet b{};
+  b.in = tmp;
+  b.out = out;
+  b.nlines = g.nlines * int64_t(n1);
+  b.nb = g.nb + 1;
+  int64_t tstride = int64_t(2 * n1 * n2);
+  for (int d = g.nb - 1; d >= 0; --d) {
+    b.bdim[d] = g.bdim[d];
+    b.bsi[d] = tstride;
+    b.bso[d] = bso[d];
+    tstride *= g.bdim[d];
+  }
+  b.bdim[g.nb] = int64_t(n1);
+  b.bsi[g.nb] = 2 * int64_t(n2);
+  b.bso[g.nb] = eso_line;
+  b.esi = 2;
+  b.eso = eso_line * int64_t(n1);
+  return b;
+}
+
+// contiguous complex line set with g's batch shape, lines of length len
+void contiguous_like(const LineSet &g, uint64_t len, int64_t *bs) {
+  int64_t st = int64_t(2 * len);
+  for (int d = g.nb - 1; d >= 0; --d) {
+    bs[d] = st;
+    st *= g.bdim[d];
+  }
+}
+
+}  // namespace
+
+size_t c2c_tmp_bytes(const C2CPlan &p, int64_t nlines) {
+  const size_t elem = p.dbl ? 16 : 8;
+  if (p.kind == kTwoPass) return size_t(nlines) * p.n1 * p.n2 * elem;
+  if (p.kind == kBlueTwoPass) return 2 * size_t(nlines) * p.n1 * p.n2 * elem;
+  return 0;
+}
+
+void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStream_t s,
+                void *tmp) {
+  const bool dbl = p.dbl;
+  const int sub_conj = dir > 0 ? SUB_CONJ : 0;
+  if (p.kind == kDirect) {
+    PassSpec ps;
+    ps.d = &p.main;
+    ps.op = OP_C2C;
+    ps.dir = dir;
+    ps.fct = fct;
+    ps.nin = ps.nout = int(p.n);
+    run_pass_any(dbl, ps, g, s);
+  } else if (p.kind == kBlueDirect) {
+    PassSpec ps;
+    ps.d = &p.main;
+    ps.op = OP_BLUE;
+    ps.dir = -1;  // forward FFT, kernel product, backward FFT (H:2732-2762)
+    ps.sub = sub_conj;
+    ps.fct = fct;
+    ps.bign = int64_t(p.n);
+    ps.t0 = p.bk.p;
+    ps.t1 = p.bkf.p;
+    ps.nin = int(p.bn2);
+    ps.nout = int(p.n);
+    run_pass_any(dbl, ps, g, s);
+  } else {
+    const uint64_t n1 = p.n1, n2 = p.n2, N = n1 * n2;
+    const int lines = dbl ? 4 : 8;
+    if (p.kind == kTwoPass) {
+      PassSpec a;
+      a.d = &p.pa;
+      a.op = OP_PASSA;
+      a.dir = dir;
+      a.other = int64_t(n2);
+      a.shift = p.shift;
+      a.t0 = p.big_lo.p;
+      a.t1 = p.big_hi.p;
+      a.nin = a.nout = int(n1);
+      a.lines = lines;
+      run_pass_any(dbl, a, pass_a_lines(g, n1, n2, g.in, g.esi, tmp), s);
+      PassSpec b;
+      b.d = &p.pb;
+      b.op = OP_PASSB;
+      b.dir = dir;
+      b.fct = fct;
+      b.other = int64_t(n1);
+      b.nin = b.nout = int(n2);
+      b.lines = lines;
+      run_pass_any(dbl, b, pass_b_lines(g, n1, n2, tmp, g.out, g.eso, g.bso), s);
+    } else {  // Bluestein in four passes through two temporaries
+      char *t1 = static_cast<char *>(tmp);
+      char *t2 = t1 + size_t(g.nlines) * N * (dbl ? 16 : 8);
+      LineSet gc{};  // contiguous work lines of length N, g's batch shape
+      gc.nlines = g.nlines;
+      gc.nb = g.nb;
+      for (int d = 0; d < g.nb; ++d) gc.bdim[d] = g.bdim[d];
+      contiguous_like(g, N, gc.bsi);
+      contiguous_like(g, N, gc.bso);
+      gc.esi = gc.eso = 2;
+      PassSpec a;  // x * chirp (zero beyond n), forward columns
+      a.d = &p.pa;
+      a.op = OP_PASSA;
+      a.dir = -1;
+      a.sub = SUB_CHIRP | sub_conj;
+      a.bign = int64_t(p.n);
+      a.other = int64_t(n2);
+      a.shift = p.shift;
+      a.t0 = p.big_lo.p;
+      a.t1 = p.big_hi.p;
+      a.t2 = p.bk.p;
+      a.nin = a.nout = int(n1);
+      a.lines = lines;
+      run_pass_any(dbl, a, pass_a_lines(g, n1, n2, g.in, g.esi, t1), s);
+      PassSpec b;  // forward rows, times the kernel, natural order into t2
+      b.d = &p.pb;
+      b.op = OP_PASSB;
+      b.dir = -1;
+      b.sub = SUB_KERNEL | sub_conj;
+      b.other = int64_t(n1);
+      b.t1 = p.bkf.p;
+      b.nin = b.nout = int(n2);
+      b.lines = lines;
+      run_pass_any(dbl, b, pass_b_lines(gc, n1, n2, t1, t2, 2, gc.bso), s);
+      PassSpec a3 = a;  // backward columns
+      a3.dir = +1;
+      a3.sub = 0;
+      a3.t2 = nullptr;
+      run_pass_any(dbl, a3, pass_a_lines(gc, n1, n2, t2, 2, t1), s);
+      PassSpec b4;  // backward rows, times chirp and fct, first n outputs
+      b4.d = &p.pb;
+      b4.op = OP_PASSB;
+      b4.dir = +1;
+      b4.sub = SUB_FINAL | sub_conj;
+      b4.fct = fct;
+      b4.bign = int64_t(p.n);
+      b4.other = int64_t(n1);
+      b4.t0 = p.bk.p;
+      b4.nin = b4.nout = int(n2);
+      b4.lines = lines;
+      run_pass_any(dbl, b4, pass_b_lines(g, n1, n2, t1, g.out, g.eso, g.bso), s);
+    }
+  }
+}
+
+void launch_real_fused(const RealPlan &rp, RealOp op, const LineSet &g, bool conj_flag,
+                       double fct, hipStream_t s) {
+  PassSpec ps;
+  ps.d = &rp.d;
+  ps.fct = fct;
+  ps.sub = conj_flag ? SUB_CONJ : 0;
+  ps.t0 = rp.rtw.p;
+  const int64_t n = int64_t(rp.n);
+  const bool even = (n % 2) == 0;
+  if (op == kR2C) {
+    ps.dir = -1;
+    ps.op = even ? OP_R2C_EVEN : OP_R2C_ODD;
+    ps.nin = int(even ? n / 2 : n);
+    ps.nout = int(n / 2 + 1);
+    ps.real_in = true;
+  } else {
+    ps.dir = +1;
+    ps.op = even ? OP_C2R_EVEN : OP_C2R_ODD;
+    ps.nin = int(even ? n / 2 : n);
+    ps.nout = int(even ? n / 2 : n);
+    ps.real_out = true;
+  }
+  run_pass_any(rp.dbl, ps, g, s);
+}
+
+// ------------------------------------------------------- pre / post kernels
+
+namespace {
+
+template <typename T>
+struct PrePostArgs {
+  int64_t nlines;
+  int nb;
+  int64_t bdim[kMaxBatchDims], bs[kMaxBatchDims];
+  int64_t es;
+  const T *rin;   // pre: real / half-complex input line set
+  T *rout;        // post: real / complex output line set
+  C2<T> *tmp;     // contiguous complex work lines of length m
+  int64_t n, m, count;
+  int mode, ortho, conj, flip;
+  T fct;
+};
+
+template <typename T>
+__device__ __forceinline__ C2<T> expipi(int64_t num, int64_t den) {
+  // e^{i pi num/den}, argument reduced in integers first
+  const int64_t per = 2 * den;
+  int64_t r = num % per;
+  if (r < 0) r += per;
+  double s, c;
+  sincospi(double(r) / double(den), &s, &c);
+  return C2<T>{T(c), T(s)};
+}
+
+template <typename T>
+__device__ __forceinline__ int64_t line_offset(const PrePostArgs<T> &a, int64_t b) {
+  int64_t rem = b, o = 0;
+  for (int d = a.nb - 1; d >= 0; --d) {
+    const int64_t q = rem / a.bdim[d];
+    o += (rem - q * a.bdim[d]) * a.bs[d];
+    rem = q;
+  }
+  return o;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_pre(const PrePostArgs<T> a) {
+  const T sqrt2 = T(1.41421356237309504880168872420969808);
+  for (int64_t b = blockIdx.y; b < a.nlines; b += gridDim.y) {
+    const T *x = a.rin + line_offset(a, b);
+    C2<T> *t = a.tmp + b * a.m;
+    const int64_t n = a.n;
+    for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < a.m;
+         j += int64_t(gridDim.x) * blockDim.x) {
+      C2<T> v = {T(0), T(0)};
+      switch (a.mode) {
+        case kPreRealAsComplex:
+          if (j < n) {
+            T xv = x[j * a.es];
+            if (a.flip && j >= 2 && (j % 2) == 0) xv = -xv;
+            v.x = xv;
+          }
+          break;
+        case kPreHermitian: {  // half complex input X[0..n/2]
+          const int64_t h = n / 2;
+          if (j == 0) {
+            v = C2<T>{x[0], T(0)};
+          } else if (j < n - h || (2 * j < n)) {
+            v = C2<T>{x[j * a.es], x[j * a.es + 1]};
+            if (a.conj) v.y = -v.y;
+          } else if (2 * j == n) {
+            v = C2<T>{x[j * a.es], T(0)};
+          } else {
+            const int64_t k = n - j;
+            v = C2<T>{x[k * a.es], x[k * a.es + 1]};
+            if (!a.conj) v.y = -v.y;
+          }
+          break;
+        }
+        case kPreDct1: {
+          const int64_t src = j < n ? j : a.m - j;
+          T xv = x[src * a.es];
+          if (a.ortho && (src == 0 || src == n - 1)) xv *= sqrt2;
+          v.x = xv;
+          break;
+        }
+        case kPreDst1:
+          if (j >= 1 && j <= n) v.x = x[(j - 1) * a.es];
+          else if (j >= n + 2) v.x = -x[(a.m - 1 - j) * a.es];
+          break;
+        case kPreType2:
+          if (j < n) v.x = x[j * a.es];
+          break;
+        case kPreDct3:
+          if (j < n) {
+            T xv = x[j * a.es];
+            if (j == 0 && a.ortho) xv *= sqrt2;
+            const T w = j == 0 ? T(1) : T(2);
+            v = (w * xv) * expipi<T>(j, 2 * n);
+          }
+          break;
+        case kPreDst3:
+          if (j >= 1 && j <= n) {
+            T xv = x[(j - 1) * a.es];
+            if (j == 1 && a.ortho) xv *= sqrt2;
+            const T w = j == n ? T(1) : T(2);
+            v = (w * xv) * expipi<T>(j, 2 * n);
+          }
+          break;
+        case kPreType4:
+          if (j < n) v = x[j * a.es] * expipi<T>(-j, 2 * n);
+          break;
+        case kPreHalfcomplex: {
+          if (j == 0) {
+            v.x = x[0];
+          } else if (2 * j < n) {
+            v = C2<T>{x[(2 * j - 1) * a.es], x[(2 * j) * a.es]};
+            if (a.conj) v.y = -v.y;
+          } else if (2 * j == n) {
+            v.x = x[(n - 1) * a.es];
+          } else {
+            const int64_t k = n - j;
+            v = C2<T>{x[(2 * k - 1) * a.es], -x[(2 * k) * a.es]};
+            if (a.conj) v.y = -v.y;
+          }
+          break;
+        }
+      }
+      t[j] = v;
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_post(const PrePostArgs<T> a) {
+  const T isqrt2 = T(0.707106781186547524400844362104849039);
+  for (int64_t b = blockIdx.y; b < a.nlines; b += gridDim.y) {
+    T *y = a.rout + line_offset(a, b);
+    const C2<T> *t = a.tmp + b * a.m;
+    const int64_t n = a.n;
+    for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < a.count;
+         k += int64_t(gridDim.x) * blockDim.x) {
+      T r = T(0);
+      switch (a.mode) {
+        case kPostComplexHalf: {
+          C2<T> v = t[k];
+          if (a.conj) v.y = -v.y;
+          y[k * a.es] = a.fct * v.x;
+          y[k * a.es + 1] = a.fct * v.y;
+          continue;
+        }
+        case kPostReal:
+          r = t[k].x;
+          if (a.flip && k >= 2 && (k % 2) == 0) r = -r;
+          break;
+        case kPostDct1:
+          r = t[k].x;
+          if (a.ortho && (k == 0 || k == n - 1)) r *= isqrt2;
+          break;
+        case kPostDst1:
+          r = -t[k + 1].y;
+          break;
+        case kPostDct2: {
+          const C2<T> z = cmul(t[k], expipi<T>(-k, 2 * n));
+          r = T(2) * z.x;
+          if (a.ortho && k == 0) r *= isqrt2;
+          break;
+        }
+        case kPostDst2: {
+          const C2<T> z = cmul(t[k + 1], expipi<T>(-(k + 1), 2 * n));
+          r = -T(2) * z.y;
+          if (a.ortho && k == 0) r *= isqrt2;
+          break;
+        }
+        case kPostDct3:
+          r = t[k].x;
+          break;
+        case kPostDst3:
+          r = t[k].y;
+          break;
+        case kPostDct4:
+        case kPostDst4: {
+          const C2<T> z = cmul(t[k], expipi<T>(-(2 * k + 1), 4 * n));
+          r = a.mode == kPostDct4 ? T(2) * z.x : -T(2) * z.y;
+          break;
+        }
+        case kPostHartley:
+          r = t[k].x + t[k].y;
+          break;
+        case kPostPackHalfcomplex:  // (the forward sign rule flips inputs only)
+          if (k == 0) r = t[0].x;
+          else if (k % 2) r = t[(k + 1) / 2].x;
+          else r = t[k / 2].y;
+          break;
+      }
+      y[k * a.es] = a.fct * r;
+    }
+  }
+}
+
+template <typename T>
+PrePostArgs<T> prepost_args(const RealPrePost &rp, const LineSet &g, bool pre) {
+  PrePostArgs<T> a{};
+  a.nlines = g.nlines;
+  a.nb = g.nb;
+  for (int d = 0; d < g.nb; ++d) {
+    a.bdim[d] = g.bdim[d];
+    a.bs[d] = pre ? g.bsi[d] : g.bso[d];
+  }
+  a.es = pre ? g.esi : g.eso;
+  a.n = rp.n;
+  a.m = rp.m;
+  a.ortho = rp.ortho;
+  a.conj = rp.conj;
+  a.flip = rp.flip;
+  return a;
+}
+
+void grid_for(int64_t count, int64_t nlines, dim3 &grid) {
+  grid.x = unsigned(std::max<int64_t>(1, std::min<int64_t>((count + kThreads - 1) / kThreads, 64)));
+  grid.y = unsigned(std::max<int64_t>(1, std::min<int64_t>(nlines, 65535)));
+  grid.z = 1;
+}
+
+}  // namespace
+
+void launch_pre(PreMode mode, const RealPrePost &rp, const LineSet &g, bool dbl,
+                void *tmp_complex, hipStream_t s) {
+  auto go = [&](auto zero) {
+    using T = decltype(zero);
+    PrePostArgs<T> a = prepost_args<T>(rp, g, true);
+    a.rin = static_cast<const T *>(g.in);
+    a.tmp = static_cast<C2<T> *>(tmp_complex);
+    a.mode = mode;
+    dim3 grid;
+    grid_for(rp.m, g.nlines, grid);
+    hipLaunchKernelGGL(k_pre<T>, grid, dim3(kThreads), 0, s, a);
+    HIP_CHECK(hipGetLastError());
+  };
+  if (dbl)
+    go(double(0));
+  else
+    go(float(0));
+}
+
+void launch_post(PostMode mode, const RealPrePost &rp, const LineSet &g, bool dbl,
+                 const void *tmp_complex, double fct, hipStream_t s) {
+  auto go = [&](auto zero) {
+    using T = decltype(zero);
+    PrePostArgs<T> a = prepost_args<T>(rp, g, false);
+    a.rout = static_cast<T *>(g.out);
+    a.tmp = static_cast<C2<T> *>(const_cast<void *>(tmp_complex));
+    a.mode = mode;
+    a.fct = T(fct);
+    a.count = mode == kPostComplexHalf ? rp.n / 2 + 1 : rp.n;
+    dim3 grid;
+    grid_for(a.count, g.nlines, grid);
+    hipLaunchKernelGGL(k_post<T>, grid, dim3(kThreads), 0, s, a);
+    HIP_CHECK(hipGetLastError());
+  };
+  if (dbl)
+    go(double(0));
+  else
+    go(float(0));
+}
+
+// -------------------------------------------------------------- mirrors
+
+namespace {
+
+struct MirrorPos {
+  int64_t self, mirror;  // offsets (scalar units) of idx and rev(idx)
+  int cls;               // 0 untouched half, 1 upper half, 2 plane larger,
+                         // 3 plane smaller, 4 plane fixed point
+};
+
+__device__ __forceinline__ MirrorPos mirror_pos(const MirrorGeom &g, int64_t t,
+                                                const int64_t *str) {
+  int64_t c[kMaxDims];
+  int64_t rem = t;
+  for (int d = g.ndim - 1; d >= 0; --d) {
+    const int64_t q = rem / g.shape[d];
+    c[d] = rem - q * g.shape[d];
+    rem = q;
+  }
+  MirrorPos p{0, 0, 0};
+  int cmp = 0;  // C-order comparison of idx against rev(idx)
+  for (int d = 0; d < g.ndim; ++d) {
+    const int64_t r = g.rev[d] ? (c[d] == 0 ? 0 : g.shape[d] - c[d]) : c[d];
+    p.self += c[d] * str[d];
+    p.mirror += r * str[d];
+    if (!cmp && d != g.last && c[d] != r) cmp = c[d] > r ? 1 : -1;
+  }
+  const int64_t n = g.shape[g.last], h = n / 2 + 1, kl = c[g.last];
+  if (kl >= h) p.cls = 1;
+  else if (kl == 0 || (n % 2 == 0 && 2 * kl == n)) p.cls = cmp > 0 ? 2 : (cmp < 0 ? 3 : 4);
+  else p.cls = 0;
+  return p;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_sym_fill(const MirrorGeom g, int64_t total,
+                                                      T *out) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const MirrorPos p = mirror_pos(g, t, g.so);
+    // the reference's C-order loop out[rev(i)] = conj(out[i]) over the half
+    // region leaves: upper half = conj(mirror); in the self-mirrored planes
+    // the later (larger) position = conj(earlier), fixed points conjugated.
+    if (p.cls == 1 || p.cls == 2) {
+      out[p.self] = out[p.mirror];
+      out[p.self + 1] = -out[p.mirror + 1];
+    } else if (p.cls == 4) {
+      out[p.self + 1] = -out[p.self + 1];
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_hartley(const MirrorGeom g, int64_t total,
+                                                     const T *tmp, T *out) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const MirrorPos po = mirror_pos(g, t, g.so);
+    const MirrorPos pt = mirror_pos(g, t, g.st);
+    T r;
+    switch (po.cls) {
+      case 1:  // upper: re - im of the mirrored bin
+      case 3:  // plane, earlier position: overwritten by the later one
+        r = tmp[pt.mirror] - tmp[pt.mirror + 1];
+        break;
+      case 4:
+        r = tmp[pt.self] - tmp[pt.self + 1];
+        break;
+      default:  // untouched half and later plane position: re + im
+        r = tmp[pt.self] + tmp[pt.self + 1];
+        break;
+    }
+    out[po.self] = r;
+  }
+}
+
+}  // namespace
+
+void launch_sym_fill(const MirrorGeom &mg, bool dbl, void *out, hipStream_t s) {
+  int64_t total = 1;
+  for (int d = 0; d < mg.ndim; ++d) total *= mg.shape[d];
+  if (total <= 0) return;
+  const unsigned grid = unsigned(std::min<int64_t>((total + kThreads - 1) / kThreads, 65536));
+  if (dbl)
+    hipLaunchKernelGGL(k_sym_fill<double>, dim3(grid), dim3(kThreads), 0, s, mg, total,
+                       static_cast<double *>(out));
+  else
+    hipLaunchKernelGGL(k_sym_fill<float>, dim3(grid), dim3(kThreads), 0, s, mg, total,
+                       static_cast<float *>(out));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_hartley_combine(const MirrorGeom &mg, bool dbl, const void *tmp, void *out,
+                            hipStream_t s) {
+  int64_t total = 1;
+  for (int d = 0; d < mg.ndim; ++d) total *= mg.shape[d];
+  if (total <= 0) return;
+  const unsigned grid = unsigned(std::min<int64_t>((total + kThreads - 1) / kThreads, 65536));
+  if (dbl)
+    hipLaunchKernelGGL(k_hartley<double>, dim3(grid), dim3(kThreads), 0, s, mg, total,
+                       static_cast<const double *>(tmp), static_cast<double *>(out));
+  else
+    hipLaunchKernelGGL(k_hartley<float>, dim3(grid), dim3(kThreads), 0, s, mg, total,
+                       static_cast<const float *>(tmp), static_cast<float *>(out));
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace rfh

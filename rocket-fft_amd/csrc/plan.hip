@@ -1,0 +1,488 @@
+// plan.hip -- host plan layer of the gfx950 transform engine.
+//
+// * Integer plan facts of the reference engine, restated bit-exactly:
+//   good_size (rocket_fft/_pocketfft_hdronly.h:588-650), largest prime
+//   factor and cost model (:548-586), FFTPACK factor orders (:1750-1778,
+//   :2629-2652) and the FFTPACK-vs-Bluestein decision (:2843-2861,
+//   :2885-2903).  The GPU follows the reference's Bluestein decision and its
+//   Bluestein length n2 = good_size_cmplx(2n-1) (:2765-2767).
+// * The GPU's own radix schedule (radix 16 first, then 8/4/2, then odd
+//   primes), independent of the reference's FFTPACK order: both compute the
+//   same DFT, the schedule only changes rounding.
+// * Twiddle, chirp and Bluestein-kernel tables generated in FP64 on the host
+//   (rounded to FP32 for complex64 plans) and kept resident in HBM for the
+//   life of the process, in a thread-safe cache keyed by (device, length,
+//   precision).  There is no LRU cap: the tables are O(n) and HBM is large.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "rfh.hpp"
+
+namespace rfh {
+
+DevBuf::~DevBuf() {
+  if (p) (void)hipFree(p);
+}
+
+void DevBuf::upload(const void *host, size_t nbytes) {
+  if (p) {
+    (void)hipFree(p);
+    p = nullptr;
+  }
+  bytes = nbytes;
+  if (!nbytes) return;
+  HIP_CHECK(hipMalloc(&p, nbytes));
+  HIP_CHECK(hipMemcpy(p, host, nbytes, hipMemcpyHostToDevice));
+}
+
+// ------------------------------------------------------- reference facts
+
+uint64_t largest_prime_factor(uint64_t n) {
+  uint64_t res = 1;
+  while ((n & 1) == 0) {
+    res = 2;
+    n >>= 1;
+  }
+  for (uint64_t x = 3; x * x <= n; x += 2)
+    while (n % x == 0) {
+      res = x;
+      n /= x;
+    }
+  return n > 1 ? n : res;
+}
+
+double cost_guess(uint64_t n) {
+  const double lfp = 1.1;  // penalty of the reference for primes above 5
+  const uint64_t ni = n;
+  double result = 0.;
+  while ((n & 1) == 0) {
+    result += 2;
+    n >>= 1;
+  }
+  for (uint64_t x = 3; x * x <= n; x += 2)
+    while (n % x == 0) {
+      result += (x <= 5) ? double(x) : lfp * double(x);
+      n /= x;
+    }
+  if (n > 1) result += (n <= 5) ? double(n) : lfp * double(n);
+  return result * double(ni);
+}
+
+// Smallest 2^a 3^b 5^c 7^d 11^e >= n (n itself up to 12): enumerate the
+// 11/7/5 part, complete it with powers of two, then trade factors of 2 for
+// factors of 3 while the candidate stays even.
+uint64_t good_size_cmplx(uint64_t n) {
+  if (n <= 12) return n;
+  uint64_t best = 2 * n;
+  for (uint64_t f11 = 1; f11 < best; f11 *= 11)
+    for (uint64_t f7 = f11; f7 < best; f7 *= 7)
+      for (uint64_t f5 = f7; f5 < best; f5 *= 5) {
+        uint64_t x = f5;
+        while (x < n) x *= 2;
+        for (;;) {
+          if (x < n) {
+            x *= 3;
+          } else if (x > n) {
+            if (x < best) best = x;
+            if (x & 1) break;
+            x >>= 1;
+          } else {
+            return n;
+          }
+        }
+      }
+  return best;
+}
+
+// Same for 2^a 3^b 5^c (n itself up to 6).
+uint64_t good_size_real(uint64_t n) {
+  if (n <= 6) return n;
+  uint64_t best = 2 * n;
+  for (uint64_t f5 = 1; f5 < best; f5 *= 5) {
+    uint64_t x = f5;
+    while (x < n) x *= 2;
+    for (;;) {
+      if (x < n) {
+        x *= 3;
+      } else if (x > n) {
+        if (x < best) best = x;
+        if (x & 1) break;
+        x >>= 1;
+      } else {
+        return n;
+      }
+    }
+  }
+  return best;
+}
+
+std::vector<uint64_t> cfftp_factors(uint64_t n) {
+  std::vector<uint64_t> f;
+  if (n <= 1) return f;
+  uint64_t len = n;
+  while ((len & 7) == 0) {
+    f.push_back(8);
+    len >>= 3;
+  }
+  while ((len & 3) == 0) {
+    f.push_back(4);
+    len >>= 2;
+  }
+  if ((len & 1) == 0) {
+    len >>= 1;
+    f.push_back(2);
+    std::swap(f.front(), f.back());
+  }
+  for (uint64_t d = 3; d * d <= len; d += 2)
+    while (len % d == 0) {
+      f.push_back(d);
+      len /= d;
+    }
+  if (len > 1) f.push_back(len);
+  return f;
+}
+
+std::vector<uint64_t> rfftp_factors(uint64_t n) {
+  std::vector<uint64_t> f;
+  if (n <= 1) return f;
+  uint64_t len = n;
+  while ((len & 3) == 0) {
+    f.push_back(4);
+    len >>= 2;
+  }
+  if ((len & 1) == 0) {
+    len >>= 1;
+    f.push_back(2);
+    std::swap(f.front(), f.back());
+  }
+  for (uint64_t d = 3; d * d <= len; d += 2)
+    while (len % d == 0) {
+      f.push_back(d);
+      len /= d;
+    }
+  if (len > 1) f.push_back(len);
+  return f;
+}
+
+bool plan_bluestein(uint64_t n, bool real, uint64_t *n2) {
+  *n2 = 0;
+  if (n == 0) return false;
+  const uint64_t lpf = (n < 50) ? 0 : largest_prime_factor(n);
+  if (lpf * lpf <= n) return false;
+  const double comp1 = (real ? 0.5 : 1.0) * cost_guess(n);
+  const double comp2 = 1.5 * (2 * cost_guess(good_size_cmplx(2 * n - 1)));
+  if (comp2 < comp1) {
+    *n2 = good_size_cmplx(2 * n - 1);
+    return true;
+  }
+  return false;
+}
+
+// ------------------------------------------------------------- GPU schedule
+
+std::vector<int> gpu_radices(uint64_t n) {
+  std::vector<int> r;
+  if (n <= 1) return r;
+  int twos = 0;
+  while ((n & 1) == 0) {
+    n >>= 1;
+    ++twos;
+  }
+  while (twos >= 4) {
+    r.push_back(16);
+    twos -= 4;
+  }
+  if (twos == 3) r.push_back(8);
+  if (twos == 2) r.push_back(4);
+  if (twos == 1) r.push_back(2);
+  for (uint64_t d = 3; d * d <= n; d += 2)
+    while (n % d == 0) {
+      r.push_back(int(d));
+      n /= d;
+    }
+  if (n > 1) r.push_back(int(n));
+  return r;
+}
+
+size_t lds_limit() {
+  static size_t lim = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return size_t(65536);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+            hipSuccess ||
+        v <= 0)
+      return size_t(65536);
+    return size_t(v);
+  }();
+  return lim;
+}
+
+// Static LDS of the line kernels (per-line offsets) is reserved first.
+static size_t lds_dynamic_budget() { return lds_limit() - 8192; }
+
+uint64_t direct_max(bool dbl, int lines) {
+  const size_t elem = dbl ? 16 : 8;
+  return lds_dynamic_budget() / (2 * elem * size_t(lines));
+}
+
+// --------------------------------------------------------------- twiddles
+
+// e^{-2 pi i e / m} in FP64 with exact integer octant reduction.
+static std::complex<double> wfwd(uint64_t e, uint64_t m) {
+  e %= m;
+  const unsigned __int128 x = (unsigned __int128)e * 8u;
+  const int oct = int(x / m);
+  const uint64_t rem = uint64_t(x - (unsigned __int128)oct * m);
+  const double quarter_pi = 0.78539816339744830961566084581987572;
+  const double a = (oct & 1) ? double(m - rem) / double(m) * quarter_pi
+                             : double(rem) / double(m) * quarter_pi;
+  const double ca = std::cos(a), sa = std::sin(a);
+  double c, s;
+  switch (oct) {
+    case 0: c = ca; s = sa; break;
+    case 1: c = sa; s = ca; break;
+    case 2: c = -sa; s = ca; break;
+    case 3: c = -ca; s = sa; break;
+    case 4: c = -ca; s = -sa; break;
+    case 5: c = -sa; s = -ca; break;
+    case 6: c = sa; s = -ca; break;
+    default: c = ca; s = -sa; break;
+  }
+  return {c, -s};  // the angle above is +2 pi e/m; forward sign flips it
+}
+
+template <typename T>
+static void upload_cplx(DevBuf &b, const std::vector<std::complex<double>> &v) {
+  std::vector<T> h(2 * v.size());
+  for (size_t i = 0; i < v.size(); ++i) {
+    h[2 * i] = T(v[i].real());
+    h[2 * i + 1] = T(v[i].imag());
+  }
+  b.upload(h.data(), h.size() * sizeof(T));
+}
+
+static void upload_any(DevBuf &b, const std::vector<std::complex<double>> &v, bool dbl) {
+  if (dbl)
+    upload_cplx<double>(b, v);
+  else
+    upload_cplx<float>(b, v);
+}
+
+static bool has_fixed_kernel(int r) {
+  return r == 2 || r == 3 || r == 4 || r == 5 || r == 8 || r == 16;
+}
+
+static void make_direct(Direct &d, uint64_t n, bool dbl) {
+  std::vector<int> rad = gpu_radices(n);
+  if (rad.size() > size_t(kMaxStages)) RFH_THROW(3, "too many FFT stages");
+  d.desc = FftDesc{};
+  d.desc.n = int(n);
+  d.desc.nst = int(rad.size());
+  std::vector<std::complex<double>> tw, gen;
+  int ns = 1;
+  for (size_t s = 0; s < rad.size(); ++s) {
+    const int R = rad[s];
+    Stage &st = d.desc.st[s];
+    st.radix = R;
+    st.ns = ns;
+    st.twoff = int(tw.size());
+    st.genoff = int(gen.size());
+    if (ns > 1)
+      for (int r = 1; r < R; ++r)
+        for (int k = 0; k < ns; ++k) tw.push_back(wfwd(uint64_t(k) * r, uint64_t(ns) * R));
+    if (!has_fixed_kernel(R))
+      for (int q = 0; q < R; ++q) gen.push_back(std::conj(wfwd(q, R)));  // (cos, +sin)
+    ns *= R;
+  }
+  if (tw.empty()) tw.push_back({1, 0});
+  if (gen.empty()) gen.push_back({1, 0});
+  upload_any(d.tw, tw, dbl);
+  upload_any(d.gen, gen, dbl);
+}
+
+// Plain recursive mixed-radix DFT in FP64 (plan time only: the Bluestein
+// kernel).  Forward sign.
+static void host_dft(const std::complex<double> *x, size_t xs, std::complex<double> *y,
+                     uint64_t n, const std::vector<int> &fac, size_t fi,
+                     const std::vector<std::complex<double>> &w, uint64_t wn,
+                     std::vector<std::complex<double>> &scratch) {
+  if (n == 1) {
+    y[0] = x[0];
+    return;
+  }
+  const uint64_t p = uint64_t(fac[fi]), m = n / p, st = wn / n;
+  for (uint64_t r = 0; r < p; ++r)
+    host_dft(x + r * xs, xs * p, y + r * m, m, fac, fi + 1, w, wn, scratch);
+  std::vector<std::complex<double>> t(n);
+  for (uint64_t k = 0; k < m; ++k)
+    for (uint64_t q = 0; q < p; ++q) {
+      const uint64_t kk = k + m * q;
+      std::complex<double> acc = y[k];
+      for (uint64_t r = 1; r < p; ++r) acc += w[((r * kk) % n) * st] * y[r * m + k];
+      t[kk] = acc;
+    }
+  std::copy(t.begin(), t.end(), y);
+}
+
+static void make_bluestein(C2CPlan &p, uint64_t n, uint64_t n2) {
+  // chirp e^{+i pi m^2 / n}; the exponent m^2 mod 2n is exact integer math
+  // (the reference's recurrence coeff += 2m-1, _pocketfft_hdronly.h:2773-2780).
+  std::vector<std::complex<double>> bk(n);
+  uint64_t coeff = 0;
+  bk[0] = {1, 0};
+  for (uint64_t m = 1; m < n; ++m) {
+    coeff += 2 * m - 1;
+    if (coeff >= 2 * n) coeff -= 2 * n;
+    bk[m] = std::conj(wfwd(coeff, 2 * n));
+  }
+  std::vector<std::complex<double>> bz(n2, {0, 0}), bkf(n2);
+  const double xn2 = 1.0 / double(n2);
+  bz[0] = bk[0] * xn2;
+  for (uint64_t m = 1; m < n; ++m) bz[m] = bz[n2 - m] = bk[m] * xn2;
+  std::vector<std::complex<double>> w(n2);
+  for (uint64_t e = 0; e < n2; ++e) w[e] = wfwd(e, n2);
+  std::vector<int> fac = gpu_radices(n2);
+  std::vector<std::complex<double>> scratch;
+  host_dft(bz.data(), 1, bkf.data(), n2, fac, 0, w, n2, scratch);
+  upload_any(p.bk, bk, p.dbl);
+  upload_any(p.bkf, bkf, p.dbl);
+  p.bn2 = n2;
+}
+
+// Two-pass split N = n1 * n2 with both passes in LDS at `lines` lines per
+// group; the most balanced split wins.  0 when none exists.
+static uint64_t pick_split(uint64_t N, bool dbl, int lines) {
+  const uint64_t lim = direct_max(dbl, lines);
+  uint64_t best = 0;
+  double bestd = 1e300;
+  for (uint64_t d = 2; d <= lim && d <= N / 2; ++d) {
+    if (N % d) continue;
+    const uint64_t e = N / d;
+    if (e > lim || e < 2) continue;
+    const double dd = std::fabs(std::log(double(d)) - std::log(double(e)));
+    if (dd < bestd) {
+      bestd = dd;
+      best = d;
+    }
+  }
+  return best;
+}
+
+static int two_pass_lines(bool dbl) { return dbl ? 4 : 8; }  // 64-byte row segments
+
+static void make_two_pass(C2CPlan &p, uint64_t N) {
+  const uint64_t n1 = pick_split(N, p.dbl, two_pass_lines(p.dbl));
+  if (!n1) RFH_THROW(3, "no two-pass split for length " + std::to_string(N));
+  p.n1 = n1;
+  p.n2 = N / n1;
+  make_direct(p.pa, p.n1, p.dbl);
+  make_direct(p.pb, p.n2, p.dbl);
+  int shift = 0;
+  while ((uint64_t(1) << (2 * shift)) < N) ++shift;
+  p.shift = shift;
+  const uint64_t lo_n = uint64_t(1) << shift, hi_n = (N + lo_n - 1) / lo_n;
+  std::vector<std::complex<double>> lo(lo_n), hi(hi_n);
+  for (uint64_t i = 0; i < lo_n; ++i) lo[i] = wfwd(i, N);
+  for (uint64_t i = 0; i < hi_n; ++i) hi[i] = wfwd(i * lo_n, N);
+  upload_any(p.big_lo, lo, p.dbl);
+  upload_any(p.big_hi, hi, p.dbl);
+}
+
+static std::shared_ptr<C2CPlan> build_c2c(uint64_t n, bool dbl) {
+  auto p = std::make_shared<C2CPlan>();
+  p->n = n;
+  p->dbl = dbl;
+  uint64_t bn2 = 0;
+  bool blue = n > 1 && plan_bluestein(n, false, &bn2);
+  const uint64_t N = blue ? bn2 : n;
+  if (N <= direct_max(dbl, 1)) {
+    p->kind = blue ? kBlueDirect : kDirect;
+    make_direct(p->main, N, dbl);
+  } else if (pick_split(N, dbl, two_pass_lines(dbl))) {
+    p->kind = blue ? kBlueTwoPass : kTwoPass;
+    make_two_pass(*p, N);
+  } else if (!blue) {
+    // a long length the two-pass split cannot factor: Bluestein always can
+    blue = true;
+    bn2 = good_size_cmplx(2 * n - 1);
+    if (bn2 <= direct_max(dbl, 1)) {
+      p->kind = kBlueDirect;
+      make_direct(p->main, bn2, dbl);
+    } else {
+      p->kind = kBlueTwoPass;
+      make_two_pass(*p, bn2);
+    }
+  } else {
+    RFH_THROW(3, "unsupported transform length " + std::to_string(n));
+  }
+  if (blue) make_bluestein(*p, n, bn2);
+  return p;
+}
+
+static std::shared_ptr<RealPlan> build_real(uint64_t n, bool dbl) {
+  auto p = std::make_shared<RealPlan>();
+  p->n = n;
+  p->dbl = dbl;
+  const uint64_t m = (n % 2 == 0) ? n / 2 : n;
+  uint64_t bn2;
+  // odd lengths that the reference sends to Bluestein stay on the generic path
+  const bool blue = (n % 2 == 1) && n > 1 && plan_bluestein(n, false, &bn2);
+  if (n >= 2 && !blue && m <= direct_max(dbl, 1)) {
+    p->fused = true;
+    make_direct(p->d, m, dbl);
+    if (n % 2 == 0) {
+      std::vector<std::complex<double>> rtw(n / 2 + 1);
+      for (uint64_t k = 0; k <= n / 2; ++k) rtw[k] = wfwd(k, n);
+      upload_any(p->rtw, rtw, dbl);
+    }
+  }
+  return p;
+}
+
+// ------------------------------------------------------------------ cache
+
+namespace {
+std::mutex g_cache_mu;
+std::map<std::tuple<int, uint64_t, bool>, std::shared_ptr<C2CPlan>> g_c2c;
+std::map<std::tuple<int, uint64_t, bool>, std::shared_ptr<RealPlan>> g_real;
+}  // namespace
+
+std::shared_ptr<C2CPlan> get_c2c_plan(uint64_t n, bool dbl) {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  auto key = std::make_tuple(dev, n, dbl);
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_c2c.find(key);
+    if (it != g_c2c.end()) return it->second;
+  }
+  auto p = build_c2c(n, dbl);  // built outside the lock, re-checked below
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  auto it = g_c2c.find(key);
+  if (it != g_c2c.end()) return it->second;
+  g_c2c[key] = p;
+  return p;
+}
+
+std::shared_ptr<RealPlan> get_real_plan(uint64_t n, bool dbl) {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  auto key = std::make_tuple(dev, n, dbl);
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_real.find(key);
+    if (it != g_real.end()) return it->second;
+  }
+  auto p = build_real(n, dbl);
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  auto it = g_real.find(key);
+  if (it != g_real.end()) return it->second;
+  g_real[key] = p;
+  return p;
+}
+
+}  // namespace rfh

@@ -1,0 +1,206 @@
+// rfh.hpp -- internal declarations of the gfx950 transform engine.
+//
+// Vocabulary: a "line" is one 1-D transform of the N-D problem (the reference
+// engine's multi_iter lines, _pocketfft_hdronly.h:3272-3331).  A line set is
+// a batch of lines described by up to kMaxBatchDims batch dimensions with
+// independent input/output strides, plus an element stride inside a line.
+// All strides and offsets are in SCALAR units (float or double), so a
+// contiguous complex line has element stride 2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace rfh {
+
+constexpr int kMaxStages = 24;
+constexpr int kMaxBatchDims = 10;
+constexpr int kMaxDims = 32;
+constexpr int kMaxLinesPerGroup = 256;
+constexpr int kThreads = 256;
+
+// ------------------------------------------------------------------ errors
+
+void set_error(int code, const std::string &msg);
+struct Error {
+  int code;
+  std::string msg;
+};
+#define RFH_THROW(code, msg) throw ::rfh::Error{(code), (msg)}
+#define HIP_CHECK(expr)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      RFH_THROW(2, std::string(#expr) + ": " + hipGetErrorString(e_));               \
+  } while (0)
+
+// ------------------------------------------------------------------ plans
+
+// One Stockham stage of an LDS-resident FFT: radix, ns = product of the
+// radices before it, offsets of its twiddles W_{ns*radix}^{k*r} (r = 1..radix-1,
+// k < ns, laid out [(r-1)*ns + k]) and, for odd radices above 5, of its
+// table of roots W_radix^q (q < radix) stored as (cos, sin).
+struct Stage {
+  int radix;
+  int ns;
+  int twoff;
+  int genoff;
+};
+
+struct FftDesc {
+  int n;
+  int nst;
+  Stage st[kMaxStages];
+};
+
+// Device memory owned by a plan.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf();
+  void upload(const void *host, size_t nbytes);
+};
+
+// An FFT small enough to run entirely in LDS (two ping-pong buffers).
+struct Direct {
+  FftDesc desc{};
+  DevBuf tw;   // stage twiddles (complex, plan precision)
+  DevBuf gen;  // roots for radices 7, 11, 13 and generic odd primes
+};
+
+enum PlanKind { kDirect = 0, kBlueDirect = 1, kTwoPass = 2, kBlueTwoPass = 3 };
+
+// Complex transform plan for one length and precision on one device.
+struct C2CPlan {
+  uint64_t n = 0;
+  bool dbl = false;
+  int kind = kDirect;
+  Direct main;           // kDirect: length n; kBlueDirect: length bn2
+  uint64_t n1 = 0, n2 = 0;  // two-pass split of the executed length
+  Direct pa, pb;         // pass A (length n1, strided columns), pass B (n2)
+  DevBuf big_lo, big_hi; // W_N^e = lo[e & mask] * hi[e >> shift]  (forward sign)
+  int shift = 0;
+  uint64_t bn2 = 0;      // Bluestein length (0 = none)
+  DevBuf bk, bkf;        // chirp e^{+i pi m^2/n} (n) and DFT(chirp)/bn2 (bn2)
+};
+
+// Real-input plan used by the fused r2c / c2r kernels: even n runs a
+// complex FFT of n/2 (rtw = e^{-2 pi i k/n}, k <= n/2), odd n a complex FFT
+// of n with zero imaginary part.  fused == false when that FFT does not fit
+// in LDS; the engine then falls back to the generic real path.
+struct RealPlan {
+  uint64_t n = 0;
+  bool dbl = false;
+  bool fused = false;
+  Direct d;
+  DevBuf rtw;
+};
+
+// process-wide switch of the specialised power-of-two kernels (fast.hip)
+extern bool g_fast_enabled;
+
+std::shared_ptr<C2CPlan> get_c2c_plan(uint64_t n, bool dbl);
+std::shared_ptr<RealPlan> get_real_plan(uint64_t n, bool dbl);
+
+// Radix schedule (execution order) for an LDS-resident FFT of length n.
+std::vector<int> gpu_radices(uint64_t n);
+// Largest line length that fits the LDS budget with `lines` lines per group.
+uint64_t direct_max(bool dbl, int lines);
+size_t lds_limit();
+
+// Reference plan facts restated (bit-exact).
+uint64_t good_size_cmplx(uint64_t n);
+uint64_t good_size_real(uint64_t n);
+uint64_t largest_prime_factor(uint64_t n);
+double cost_guess(uint64_t n);
+std::vector<uint64_t> cfftp_factors(uint64_t n);
+std::vector<uint64_t> rfftp_factors(uint64_t n);
+bool plan_bluestein(uint64_t n, bool real, uint64_t *n2);
+
+// ------------------------------------------------------------------ lines
+
+struct LineSet {
+  const void *in = nullptr;  // scalar base pointers
+  void *out = nullptr;
+  int64_t nlines = 0;
+  int nb = 0;                // batch dims, innermost last
+  int64_t bdim[kMaxBatchDims] = {};
+  int64_t bsi[kMaxBatchDims] = {};
+  int64_t bso[kMaxBatchDims] = {};
+  int64_t esi = 0, eso = 0;  // element strides (scalar units)
+};
+
+// Kernel launchers (kernels.hip).  All enqueue on `s`.
+// dir: -1 forward, +1 backward.
+void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStream_t s,
+                void *tmp);
+size_t c2c_tmp_bytes(const C2CPlan &p, int64_t nlines);
+
+enum RealOp { kR2C = 0, kC2R = 1 };
+// Fused real transforms: r2c (real lines -> n/2+1 complex, conj when
+// conj_flag) or c2r (n/2+1 complex -> real lines, input conjugated when
+// conj_flag).  Requires rp.fused.
+void launch_real_fused(const RealPlan &rp, RealOp op, const LineSet &g, bool conj_flag,
+                       double fct, hipStream_t s);
+
+// Pre/post kernels of the generic real path: real (or half complex) lines
+// <-> contiguous complex lines of length m.
+enum PreMode {
+  kPreRealAsComplex = 0,  // a_j = x_j (j < n), zero beyond
+  kPreHermitian,          // full Hermitian spectrum from n/2+1 bins (c2r)
+  kPreDct1,
+  kPreDst1,
+  kPreType2,              // zero padded to 2n
+  kPreDct3,
+  kPreDst3,
+  kPreType4,
+  kPreHalfcomplex,        // FFTPACK halfcomplex -> Hermitian spectrum
+};
+enum PostMode {
+  kPostComplexHalf = 0,   // bins 0..n/2 (r2c)
+  kPostReal,              // Re
+  kPostDct1,
+  kPostDst1,
+  kPostDct2,
+  kPostDst2,
+  kPostDct3,
+  kPostDst3,
+  kPostDct4,
+  kPostDst4,
+  kPostHartley,           // Re + Im
+  kPostPackHalfcomplex,   // bins -> FFTPACK halfcomplex
+};
+struct RealPrePost {
+  int64_t n = 0;      // logical line length (real)
+  int64_t m = 0;      // complex work length
+  bool ortho = false;
+  bool conj = false;  // pre: conjugate input spectrum; post: conjugate output
+  bool flip = false;  // negate entries 2, 4, 6, ... (FFTPACK sign rule)
+};
+void launch_pre(PreMode mode, const RealPrePost &rp, const LineSet &g, bool dbl,
+                void *tmp_complex, hipStream_t s);
+void launch_post(PostMode mode, const RealPrePost &rp, const LineSet &g, bool dbl,
+                 const void *tmp_complex, double fct, hipStream_t s);
+
+// Hermitian completion of c2c_sym and the combine step of the genuine
+// Hartley transform (exact overwrite order of the reference's C-order loop).
+struct MirrorGeom {
+  int ndim = 0;
+  int64_t shape[kMaxDims] = {};
+  int64_t so[kMaxDims] = {};   // output strides (scalar units)
+  int64_t st[kMaxDims] = {};   // temp strides (scalar units; genuine Hartley)
+  unsigned char rev[kMaxDims] = {};
+  int last = 0;                // axes.back()
+};
+void launch_sym_fill(const MirrorGeom &mg, bool dbl, void *out, hipStream_t s);
+void launch_hartley_combine(const MirrorGeom &mg, bool dbl, const void *tmp, void *out,
+                            hipStream_t s);
+
+}  // namespace rfh

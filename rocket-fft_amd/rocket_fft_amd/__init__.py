@@ -1,0 +1,330 @@
+"""rocket_fft_amd -- rocket-fft's low-level transform API on MI355X (gfx950).
+
+Same functions, argument order and meaning as rocket-fft's low-level
+interface (``rocket_fft/__init__.py:18-34``, typed in
+``rocket_fft/__init__.pyi:6-105``)::
+
+    c2c(ain, aout, axes, forward, fct, nthreads)
+    r2c(ain, aout, axes, forward, fct, nthreads)
+    c2r(ain, aout, axes, forward, fct, nthreads)
+    c2c_sym(ain, aout, axes, forward, fct, nthreads)
+    dct(ain, aout, axes, type, fct, ortho, nthreads)
+    dst(ain, aout, axes, type, fct, ortho, nthreads)
+    r2r_separable_hartley(ain, aout, axes, fct, nthreads)
+    r2r_genuine_hartley(ain, aout, axes, fct, nthreads)
+    r2r_fftpack(ain, aout, axes, real2hermitian, forward, fct, nthreads)
+    good_size(n, real)
+
+Each call goes through the C ABI of ``librocketfft_hip.so`` (the same ten
+``numba_*`` symbols rocket-fft's Numba intrinsics bind, see
+``rocket_fft/pocketfft.py:32-128``), with the Numba array struct built here
+instead of by Numba.  Arrays may be NumPy arrays (host memory, staged through
+the GPU) or torch CUDA tensors (computed in place in HBM, on torch's current
+stream, asynchronously like any torch op).  There is no CPU fallback: the
+functions raise if the HIP library is missing.
+"""
+import ctypes
+import numbers
+import os
+
+import numpy as np
+
+__all__ = [
+    "c2c", "r2c", "c2r", "c2c_sym", "dct", "dst", "r2r_separable_hartley",
+    "r2r_genuine_hartley", "r2r_fftpack", "good_size", "separable_hartley",
+    "genuine_hartley", "fftpack", "library_path", "last_error", "plan_facts",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "librocketfft_hip.so")
+_lib = None
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(
+            f"rocket_fft_amd: {_LIB_PATH} is not built; run __graft_entry__.build() "
+            "or python rocket-fft_amd/rocket_fft_amd/_build.py")
+    L = ctypes.CDLL(_LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+    u64, vp, b, d = ctypes.c_uint64, ctypes.c_void_p, ctypes.c_bool, ctypes.c_double
+    L.numba_good_size.argtypes = [u64, b]
+    L.numba_good_size.restype = u64
+    for name in ("numba_c2c", "numba_r2c", "numba_c2r", "numba_c2c_sym"):
+        getattr(L, name).argtypes = [u64, vp, vp, vp, b, d, u64]
+        getattr(L, name).restype = None
+    for name in ("numba_dct", "numba_dst"):
+        getattr(L, name).argtypes = [u64, vp, vp, vp, u64, d, b, u64]
+        getattr(L, name).restype = None
+    L.numba_r2r_fftpack.argtypes = [u64, vp, vp, vp, b, b, d, u64]
+    L.numba_r2r_fftpack.restype = None
+    for name in ("numba_r2r_separable_hartley", "numba_r2r_genuine_hartley"):
+        getattr(L, name).argtypes = [u64, vp, vp, vp, d, u64]
+        getattr(L, name).restype = None
+    L.rfh_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    L.rfh_last_error.restype = ctypes.c_int
+    L.rfh_clear_error.argtypes = []
+    L.rfh_clear_error.restype = None
+    L.rfh_set_stream.argtypes = [vp, b]
+    L.rfh_set_stream.restype = ctypes.c_int
+    for name in ("rfh_cfftp_factors", "rfh_rfftp_factors"):
+        getattr(L, name).argtypes = [u64, vp, ctypes.c_int]
+        getattr(L, name).restype = ctypes.c_int
+    L.rfh_plan_choice.argtypes = [u64, b, ctypes.POINTER(u64)]
+    L.rfh_plan_choice.restype = ctypes.c_int
+    L.rfh_gpu_schedule.argtypes = [u64, b, ctypes.POINTER(ctypes.c_int), vp, ctypes.c_int,
+                                   ctypes.POINTER(u64), ctypes.POINTER(u64)]
+    L.rfh_gpu_schedule.restype = ctypes.c_int
+    L.rfh_set_fast_path.argtypes = [b]
+    L.rfh_set_fast_path.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def last_error():
+    """(code, message) of the calling thread's last failed call."""
+    buf = ctypes.create_string_buffer(512)
+    code = _load().rfh_last_error(buf, 512)
+    return code, buf.value.decode()
+
+
+# ------------------------------------------------------------ array structs
+
+_struct_cache = {}
+
+
+def _arystruct_type(ndim):
+    t = _struct_cache.get(ndim)
+    if t is None:
+        t = type(f"arystruct_{ndim}", (ctypes.Structure,), {"_fields_": [
+            ("meminfo", ctypes.c_void_p), ("parent", ctypes.c_void_p),
+            ("nitems", ctypes.c_int64), ("itemsize", ctypes.c_int64),
+            ("data", ctypes.c_void_p),
+            ("shape_and_strides", ctypes.c_int64 * (2 * max(ndim, 1)))]})
+        _struct_cache[ndim] = t
+    return t
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+_TORCH_ITEMSIZE = {"torch.float32": 4, "torch.float64": 8, "torch.complex64": 8,
+                   "torch.complex128": 16}
+
+
+def _describe(a):
+    """(ndim, shape, byte strides, itemsize, data pointer, kind, is_cuda)."""
+    if _is_torch(a):
+        key = str(a.dtype)
+        if key not in _TORCH_ITEMSIZE:
+            raise TypeError(f"unsupported tensor dtype {a.dtype}")
+        isz = _TORCH_ITEMSIZE[key]
+        kind = "c" if a.is_complex() else "f"
+        return (a.dim(), tuple(a.shape), tuple(s * isz for s in a.stride()), isz,
+                a.data_ptr(), kind, a.is_cuda)
+    if not isinstance(a, np.ndarray):
+        raise TypeError("arrays must be numpy arrays or torch tensors")
+    if a.dtype.kind not in "fc" or a.dtype.itemsize not in (4, 8, 16):
+        raise TypeError(f"unsupported array dtype {a.dtype}")
+    return (a.ndim, a.shape, a.strides, a.dtype.itemsize, a.ctypes.data, a.dtype.kind, False)
+
+
+def _struct(a):
+    nd, shape, strides, isz, ptr, kind, cuda = _describe(a)
+    S = _arystruct_type(nd)()
+    S.nitems = int(np.prod(shape)) if nd else 1
+    S.itemsize = isz
+    S.data = ptr
+    for i in range(nd):
+        S.shape_and_strides[i] = shape[i]
+        S.shape_and_strides[nd + i] = strides[i]
+    return S, nd, kind, cuda
+
+
+def _axes_struct(axes, ndim):
+    ax = np.asarray(axes)
+    if ax.ndim != 1:
+        raise TypeError("Axes must be a one-dimensional array")
+    # any integer or float axes array is copied to uint64 (pocketfft.py:155-167)
+    ax = np.ascontiguousarray(ax, dtype=np.int64)
+    if ax.size and (ax.min() < 0 or ax.max() >= ndim):
+        raise ValueError("axis out of range")
+    ax = ax.astype(np.uint64)
+    S = _arystruct_type(1)()
+    S.nitems = ax.size
+    S.itemsize = 8
+    S.data = ax.ctypes.data
+    S.shape_and_strides[0] = ax.size
+    S.shape_and_strides[1] = 8
+    return S, ax
+
+
+def _check_bool(name, v):
+    if not isinstance(v, (bool, np.bool_)):
+        raise TypeError(f"'{name}' must be a boolean")
+
+
+def _check_int(name, v):
+    if isinstance(v, (bool, np.bool_)) or not isinstance(v, (numbers.Integral, np.integer)):
+        raise TypeError(f"'{name}' must be an integer")
+
+
+def _check_float(name, v):
+    # rocket_fft/pocketfft.py widens float arguments only; integer fct fails
+    # typing there (tests/test_low_level_interface.py:218-225)
+    if isinstance(v, (bool, np.bool_, numbers.Integral, np.integer)) or not isinstance(
+            v, (numbers.Real, np.floating)):
+        raise TypeError(f"'{name}' must be a floating-point number")
+
+
+def _call(fname, ain, aout, axes, *mid):
+    S_in, nd, kin, cin = _struct(ain)
+    S_out, nd2, kout, cout = _struct(aout)
+    if nd != nd2:
+        raise TypeError("Input and output array must have the same number of dimensions")
+    S_ax, keep = _axes_struct(axes, nd)
+    L = _load()
+    stream = None
+    if cin or cout:
+        import torch
+        stream = torch.cuda.current_stream(aout.device if cout else ain.device).cuda_stream
+        L.rfh_set_stream(ctypes.c_void_p(stream), False)
+    try:
+        L.rfh_clear_error()
+        getattr(L, fname)(nd, ctypes.byref(S_in), ctypes.byref(S_out), ctypes.byref(S_ax), *mid)
+        code, msg = last_error()
+    finally:
+        if stream is not None:
+            L.rfh_set_stream(None, True)
+    del keep
+    if code:
+        raise RuntimeError(f"{fname}: {msg}")
+    return aout
+
+
+def _cmplx_args(forward, fct, nthreads):
+    _check_bool("forward", forward)
+    _check_float("fct", fct)
+    _check_int("nthreads", nthreads)
+    return bool(forward), float(fct), int(nthreads)
+
+
+def c2c(ain, aout, axes, forward, fct, nthreads=1):
+    """Complex-to-complex DFT over ``axes`` (_pocketfft_numba.cpp:31-49)."""
+    f, s, t = _cmplx_args(forward, fct, nthreads)
+    return _call("numba_c2c", ain, aout, axes, f, s, t)
+
+
+def r2c(ain, aout, axes, forward, fct, nthreads=1):
+    """Real-to-half-complex DFT (_pocketfft_numba.cpp:91-109)."""
+    f, s, t = _cmplx_args(forward, fct, nthreads)
+    return _call("numba_r2c", ain, aout, axes, f, s, t)
+
+
+def c2r(ain, aout, axes, forward, fct, nthreads=1):
+    """Half-complex-to-real DFT; shape from ``aout`` (_pocketfft_numba.cpp:145-163)."""
+    f, s, t = _cmplx_args(forward, fct, nthreads)
+    return _call("numba_c2r", ain, aout, axes, f, s, t)
+
+
+def c2c_sym(ain, aout, axes, forward, fct, nthreads=1):
+    """Real-to-full-complex DFT (_pocketfft_numba.cpp:111-143)."""
+    f, s, t = _cmplx_args(forward, fct, nthreads)
+    return _call("numba_c2c_sym", ain, aout, axes, f, s, t)
+
+
+def _real_args(type, fct, ortho, nthreads):
+    _check_int("type", type)
+    _check_float("fct", fct)
+    _check_bool("ortho", ortho)
+    _check_int("nthreads", nthreads)
+    if type < 1 or type > 4:
+        raise ValueError("invalid DCT/DST type")
+    return int(type), float(fct), bool(ortho), int(nthreads)
+
+
+def dct(ain, aout, axes, type, fct, ortho, nthreads=1):
+    """DCT type 1-4 (_pocketfft_numba.cpp:51-69)."""
+    return _call("numba_dct", ain, aout, axes, *_real_args(type, fct, ortho, nthreads))
+
+
+def dst(ain, aout, axes, type, fct, ortho, nthreads=1):
+    """DST type 1-4 (_pocketfft_numba.cpp:71-89)."""
+    return _call("numba_dst", ain, aout, axes, *_real_args(type, fct, ortho, nthreads))
+
+
+def r2r_separable_hartley(ain, aout, axes, fct, nthreads=1):
+    """Hartley transform axis by axis (_pocketfft_numba.cpp:185-203)."""
+    _check_float("fct", fct)
+    _check_int("nthreads", nthreads)
+    return _call("numba_r2r_separable_hartley", ain, aout, axes, float(fct), int(nthreads))
+
+
+def r2r_genuine_hartley(ain, aout, axes, fct, nthreads=1):
+    """N-D Hartley transform (_pocketfft_numba.cpp:205-223)."""
+    _check_float("fct", fct)
+    _check_int("nthreads", nthreads)
+    return _call("numba_r2r_genuine_hartley", ain, aout, axes, float(fct), int(nthreads))
+
+
+def r2r_fftpack(ain, aout, axes, real2hermitian, forward, fct, nthreads=1):
+    """FFTPACK halfcomplex transforms (_pocketfft_numba.cpp:165-183)."""
+    _check_bool("real2hermitian", real2hermitian)
+    f, s, t = _cmplx_args(forward, fct, nthreads)
+    return _call("numba_r2r_fftpack", ain, aout, axes, bool(real2hermitian), f, s, t)
+
+
+def good_size(n, real=False):
+    """Smallest fast length >= n (_pocketfft_numba.cpp:25-29); bit-exact."""
+    if not isinstance(n, (numbers.Integral, np.integer, bool, np.bool_)):
+        raise TypeError("The first argument 'n' must be an integer")
+    if not isinstance(real, (numbers.Integral, np.integer, bool, np.bool_)):
+        raise TypeError("The second argument 'real' must be a boolean")
+    return int(_load().numba_good_size(int(n), bool(real)))
+
+
+separable_hartley = r2r_separable_hartley
+genuine_hartley = r2r_genuine_hartley
+fftpack = r2r_fftpack
+
+
+def plan_facts(n):
+    """Reference plan facts for length n: complex and real FFTPACK factor lists
+    and the Bluestein decisions (with n2)."""
+    L = _load()
+    buf = np.zeros(64, dtype=np.uint64)
+    k = L.rfh_cfftp_factors(int(n), buf.ctypes.data, 64)
+    cf = [int(x) for x in buf[:k]]
+    k = L.rfh_rfftp_factors(int(n), buf.ctypes.data, 64)
+    rf = [int(x) for x in buf[:k]]
+    n2c, n2r = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    bc = L.rfh_plan_choice(int(n), False, ctypes.byref(n2c))
+    br = L.rfh_plan_choice(int(n), True, ctypes.byref(n2r))
+    return {"cfftp": cf, "rfftp": rf, "blue_c": (bool(bc), int(n2c.value)),
+            "blue_r": (bool(br), int(n2r.value))}
+
+
+def set_fast_path(enabled):
+    """Process-wide switch of the specialised power-of-two kernels (tests)."""
+    _load().rfh_set_fast_path(bool(enabled))
+
+
+def gpu_schedule(n, double=False):
+    """(kind, radices, n1, n2) of the GPU plan for a complex transform of n
+    (needs a GPU: plans live in device memory)."""
+    L = _load()
+    kind = ctypes.c_int(0)
+    rad = (ctypes.c_int * 32)()
+    n1, n2 = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    k = L.rfh_gpu_schedule(int(n), bool(double), ctypes.byref(kind), ctypes.addressof(rad), 32,
+                           ctypes.byref(n1), ctypes.byref(n2))
+    if k < 0:
+        raise RuntimeError(last_error()[1])
+    return kind.value, list(rad[:k]), n1.value, n2.value

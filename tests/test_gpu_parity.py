@@ -1,0 +1,210 @@
+"""GPU parity: every low-level entry point through the C ABI against the
+oracle (long-double restatement) and, where it was built, the reference
+engine itself.  Bar: rel-L2 <= 1e-13 (complex128/float64), <= 1e-6
+(complex64/float32).  Inputs are seeded N(0,1)."""
+import numpy as np
+import pytest
+
+import rocket_fft_amd as R
+from helpers import (CTYPE, REF, crandn, on_gpu, oracle_out, ref_out, rel_l2, rrandn,
+                     tol_of)
+
+pytestmark = pytest.mark.gpu
+
+
+def check(fname, ain, aout_like, *args, inplace=False, tol=None):
+    print(f"case {fname} in={ain.shape}{ain.dtype} axes/args={args}", flush=True)
+    got = on_gpu(R, fname, ain, aout_like, *args, inplace=inplace)
+    tol = tol if tol is not None else tol_of(aout_like.dtype)
+    want = oracle_out(fname, ain, aout_like, *args)
+    e = rel_l2(got, want)
+    assert e <= tol, f"{fname} vs oracle: rel-L2 {e:.3e} > {tol:.0e}"
+    if REF.available():
+        e2 = rel_l2(got, ref_out(fname, ain, aout_like, *args))
+        assert e2 <= tol, f"{fname} vs reference: rel-L2 {e2:.3e} > {tol:.0e}"
+    return e
+
+
+C2C_LENGTHS = [1, 2, 3, 4, 5, 7, 8, 11, 13, 16, 17, 30, 47, 64, 97, 100, 128, 243, 256,
+               343, 512, 1000, 1024, 1536, 2187, 4096, 4913, 8192, 4099, 8232, 16384]
+
+
+@pytest.mark.parametrize("n", C2C_LENGTHS)
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_c2c_lengths(n, dtype):
+    rs = np.random.RandomState(n)
+    x = crandn(rs, (3, n), dtype)
+    check("c2c", x, x, np.array([1]), True, 1.0)
+    check("c2c", x, x, np.array([1]), False, 1.0 / n)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_c2c_two_pass_long(dtype):
+    rs = np.random.RandomState(7)
+    x = crandn(rs, (2, 65536), dtype)
+    check("c2c", x, x, np.array([1]), True, 1.0)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_c2c_bluestein_two_pass(dtype):
+    rs = np.random.RandomState(8)
+    x = crandn(rs, (2, 65537), dtype)
+    check("c2c", x, x, np.array([1]), True, 1.0, tol=3 * tol_of(dtype))
+    check("c2c", x, x, np.array([1]), False, 1.0 / 65537, tol=3 * tol_of(dtype))
+
+
+@pytest.mark.parametrize("shape,axes", [((16, 24, 10), (0, 1, 2)), ((16, 24, 10), (2, 0)),
+                                        ((33, 17), (0,)), ((33, 17), (1, 0)),
+                                        ((5, 6, 7, 8), (3, 1)), ((64, 3), (0,))])
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_c2c_axes(shape, axes, dtype):
+    rs = np.random.RandomState(1)
+    x = crandn(rs, shape, dtype)
+    check("c2c", x, x, np.array(axes), True, 0.5)
+    check("c2c", x, x, np.array(axes), False, 1.0, inplace=True)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 9, 15, 16, 30, 64, 101, 128, 255, 256,
+                               512, 1000, 4097, 4099, 8192])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_r2c_c2r_lengths(n, dtype):
+    rs = np.random.RandomState(n + 3)
+    x = rrandn(rs, (4, n), dtype)
+    h = np.zeros((4, n // 2 + 1), CTYPE[dtype])
+    check("r2c", x, h, np.array([1]), True, 1.0)
+    check("r2c", x, h, np.array([1]), False, 2.0)
+    X = crandn(rs, (4, n // 2 + 1), CTYPE[dtype])
+    check("c2r", X, x, np.array([1]), False, 1.0 / n)
+    check("c2r", X, x, np.array([1]), True, 1.0)
+
+
+@pytest.mark.parametrize("shape,axes", [((8, 6, 10), (0, 1, 2)), ((8, 6, 10), (2, 0)),
+                                        ((9, 7), (0, 1)), ((12, 5), (0,)), ((4, 4, 4), (1,))])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_r2c_c2r_axes(shape, axes, dtype):
+    rs = np.random.RandomState(2)
+    x = rrandn(rs, shape, dtype)
+    hs = list(shape)
+    hs[axes[-1]] = shape[axes[-1]] // 2 + 1
+    h = np.zeros(hs, CTYPE[dtype])
+    check("r2c", x, h, np.array(axes), True, 1.0)
+    X = crandn(rs, hs, CTYPE[dtype])
+    check("c2r", X, x, np.array(axes), False, 1.0)
+    full = np.zeros(shape, CTYPE[dtype])
+    check("c2c_sym", x, full, np.array(axes), True, 1.0)
+    check("c2c_sym", x, full, np.array(axes), False, 0.5)
+
+
+@pytest.mark.parametrize("kind", ["dct", "dst"])
+@pytest.mark.parametrize("type", [1, 2, 3, 4])
+@pytest.mark.parametrize("ortho", [False, True])
+@pytest.mark.parametrize("n", [2, 3, 8, 15, 64, 100])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_dct_dst(kind, type, ortho, n, dtype):
+    rs = np.random.RandomState(n)
+    x = rrandn(rs, (5, n), dtype)
+    check(kind, x, x, np.array([1]), type, 1.0, ortho)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_dct2_ortho_2d(dtype):
+    rs = np.random.RandomState(2)
+    x = rs.uniform(-1, 1, (64, 48)).astype(dtype)
+    check("dct", x, x, np.array([0, 1]), 2, 1.0 / np.sqrt(4 * 64 * 48), True)
+    check("dst", x, x, np.array([1, 0]), 3, 1.0, True, inplace=True)
+
+
+@pytest.mark.parametrize("r2h", [False, True])
+@pytest.mark.parametrize("fwd", [False, True])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 7, 10, 64])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_fftpack(r2h, fwd, n, dtype):
+    rs = np.random.RandomState(n)
+    x = rrandn(rs, (3, n), dtype)
+    check("r2r_fftpack", x, x, np.array([1]), r2h, fwd, 1.0)
+
+
+@pytest.mark.parametrize("shape,axes", [((10,), (0,)), ((127,), (0,)), ((16, 17), (0, 1)),
+                                        ((1, 9), (0, 1)), ((9, 1), (0, 1)),
+                                        ((6, 5, 7), (0, 1, 2)), ((6, 5, 7), (2, 1))])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_hartley(shape, axes, dtype):
+    rs = np.random.RandomState(4)
+    x = rrandn(rs, shape, dtype)
+    check("r2r_separable_hartley", x, x, np.array(axes), 1.0)
+    check("r2r_genuine_hartley", x, x, np.array(axes), 1.0)
+
+
+def test_host_arrays_roundtrip():
+    """NumPy (host) arrays go through the staging path."""
+    rs = np.random.RandomState(5)
+    x = crandn(rs, (8, 100), np.complex128)
+    y = np.zeros_like(x)
+    R.c2c(x, y, np.array([1]), True, 1.0, 1)
+    assert rel_l2(y, oracle_out("c2c", x, x, np.array([1]), True, 1.0)) < 1e-13
+    z = np.zeros_like(x)
+    R.c2c(y, z, np.array([1]), False, 1.0 / 100, 1)
+    assert rel_l2(z, x) < 1e-13
+
+
+def test_strided_views():
+    """Non-contiguous and negative-stride views (T/test_numpy_testsuite.py:323-351):
+    device tensors with a step, and host arrays with negative strides."""
+    import torch
+    rs = np.random.RandomState(6)
+    base = crandn(rs, (20, 40), np.complex128)
+    tb = torch.from_numpy(base).cuda()
+    view = tb[::2, 1::3]
+    out = torch.zeros(view.shape, dtype=torch.complex128, device="cuda")
+    R.c2c(view, out, np.array([0, 1]), True, 1.0)
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), np.fft.fftn(base[::2, 1::3])) < 1e-13
+    hv = base[::2, ::-3]
+    ho = np.zeros(hv.shape, np.complex128)[::-1]
+    R.c2c(hv, ho, np.array([0, 1]), True, 1.0)
+    assert rel_l2(ho, np.fft.fftn(hv)) < 1e-13
+
+
+def test_zero_size_noop():
+    import torch
+    x = torch.zeros((0, 8), dtype=torch.complex64, device="cuda")
+    R.c2c(x, x, np.array([1]), True, 1.0)
+
+
+def test_schedule_report():
+    """Prints the GPU schedule of the configuration lengths (informational)."""
+    for n in (256, 512, 1000, 1536, 2187, 4096, 4913, 4099, 8232, 65537, 1 << 20):
+        for dbl in (False, True):
+            print(n, "c128" if dbl else "c64", R.gpu_schedule(n, dbl), flush=True)
+
+
+@pytest.mark.parametrize("fast", [True, False])
+@pytest.mark.parametrize("n", [2, 8, 32, 64, 128, 256, 512, 1024, 2048, 4096])
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_fast_and_generic_paths(fast, n, dtype):
+    """Power-of-two lengths through the specialised kernels and, switched
+    off, through the generic ones: contiguous lines, interleaved lines
+    (axis 0), r2c/c2r on the last axis and on axis 0."""
+    R.set_fast_path(fast)
+    try:
+        rs = np.random.RandomState(n)
+        x = crandn(rs, (n, 24), dtype)
+        check("c2c", x, x, np.array([0]), True, 1.0)
+        check("c2c", x.T.copy(), x.T, np.array([1]), False, 0.5)
+        rt = np.float32 if dtype == np.complex64 else np.float64
+        xr = rrandn(rs, (6, 2 * n), rt)
+        check("r2c", xr, np.zeros((6, n + 1), dtype), np.array([1]), True, 1.0)
+        check("c2r", crandn(rs, (6, n + 1), dtype), xr, np.array([1]), True, 1.0)
+        xr0 = rrandn(rs, (2 * n, 10), rt)
+        check("r2c", xr0, np.zeros((n + 1, 10), dtype), np.array([0]), False, 1.0)
+        check("c2r", crandn(rs, (n + 1, 10), dtype), xr0, np.array([0]), False, 1.0)
+    finally:
+        R.set_fast_path(True)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_two_pass_pow2_fast(dtype):
+    rs = np.random.RandomState(11)
+    x = crandn(rs, (2, 1 << 18), dtype)
+    check("c2c", x, x, np.array([1]), True, 1.0)
+    check("c2c", x, x, np.array([1]), False, 1.0 / (1 << 18))
