@@ -123,6 +123,67 @@ def extra_configs(torch, R, stream, reps):
     return out
 
 
+def probe(torch, R):
+    """Calibration copy (reads and writes BYTES_C2C_PASS/2 each) then the
+    dominant kernel, three launches per strided axis, no graph."""
+    X = torch.randn((N3, N3, N3 // 2 + 1), dtype=torch.complex64, device="cuda")
+    Y = torch.empty_like(X)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        Y.copy_(X)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        R.c2c(X, X, np.array([1]), True, 1.0, 1)
+        R.c2c(X, X, np.array([0]), True, 1.0, 1)
+    torch.cuda.synchronize()
+    print("probe done", flush=True)
+
+
+def ragged_config(torch, R, stream):
+    """4096 transforms of log-uniform lengths 64..65536 (70% good sizes, 30%
+    raw incl. primes; 60% c2c complex64, 40% r2c float32) in one persistent
+    launch, against one drop-in call per transform."""
+    spec = R.ragged_workload(njobs=4096, seed=5)
+    jobs, nbytes = [], 0
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    for kind, n in spec:
+        if kind == "c2c":
+            x = torch.randn(n, dtype=torch.complex64, device="cuda", generator=g)
+            y = torch.empty(n, dtype=torch.complex64, device="cuda")
+            nbytes += 16 * n
+        else:
+            x = torch.randn(n, dtype=torch.float32, device="cuda", generator=g)
+            y = torch.empty(n // 2 + 1, dtype=torch.complex64, device="cuda")
+            nbytes += 4 * n + 8 * (n // 2 + 1)
+        jobs.append((kind, True, n, x, y, 1.0))
+    with torch.cuda.stream(stream):
+        b = R.RaggedBatch(jobs, double=False, autotune=True)
+        b.execute_serial()
+        b.execute_serial()
+    torch.cuda.synchronize()
+    ref = [j[4].clone() for j in jobs]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        e0.record(stream)
+        for _ in range(3):
+            b.execute_serial()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    serial_ms = e0.elapsed_time(e1) / 3
+    with torch.cuda.stream(stream):
+        b.execute()
+    torch.cuda.synchronize()
+    ms, _ = graph_time(torch, b.execute, stream, 10)
+    worst = max(float(torch.linalg.norm(r - j[4]) / torch.linalg.norm(r)) for r, j in zip(ref, jobs))
+    info = b.info()
+    b.close()
+    return {"jobs": len(jobs), "bytes": nbytes, "persistent_ms": round(ms, 4),
+            "GB/s": round(nbytes / ms / 1e6, 1), "hbm_frac": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 3),
+            "per_transform_ms": round(serial_ms, 3), "speedup_vs_per_transform": round(serial_ms / ms, 2),
+            "max_rel_l2_persistent_vs_per_transform": worst, "autotune": info}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +192,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="only launch the dominant kernel (3x per strided axis) after a "
+                         "3x device copy of the same 135 MB array, for counter runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: every transform of this workload fits one "
@@ -141,6 +205,9 @@ def main():
     import torch
     import rocket_fft_amd as R
 
+    if args.probe:
+        probe(torch, R)
+        return
     x_h, d_h = inputs()
     dev = torch.device("cuda:0")
     x = torch.from_numpy(x_h).to(dev)
@@ -237,6 +304,7 @@ def main():
     }
     if not args.no_extra:
         res["configs"] = extra_configs(torch, R, stream, 10)
+        res["configs"]["ragged_4096_c64_r2c"] = ragged_config(torch, R, stream)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(x_h, d_h, args.cpu_seconds)
     print(json.dumps(res))

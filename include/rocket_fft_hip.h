@@ -111,6 +111,35 @@ int rfh_set_stream(void *stream, bool blocking);
  * kernels.  Process-wide; meant for tests that compare the two paths. */
 int rfh_set_fast_path(bool enabled);
 
+/* ---- ragged batches ---------------------------------------------------------
+ * Many independent transforms of uneven lengths and kinds (complex c2c, real
+ * r2c / c2r), planned once and executed in one persistent launch (a job
+ * queue drained by resident workgroups, largest job first).  No reference
+ * counterpart: rocket-fft calls its engine once per array.  Each job is one
+ * contiguous line in device memory; fct, forward and the r2c/c2r
+ * conjugation rules mean what they mean in numba_c2c / numba_r2c /
+ * numba_c2r.  Execution allocates nothing and can be captured in a HIP
+ * graph; the buffers named at creation are the ones every execution uses.
+ * autotune = true times LDS budgets and grid sizes on the batch and keeps
+ * the fastest (remembered per batch signature for the process). */
+enum { RFH_JOB_C2C = 0, RFH_JOB_R2C = 1, RFH_JOB_C2R = 2 };
+typedef struct {
+    int32_t kind;     /* RFH_JOB_* */
+    int32_t forward;
+    uint64_t n;       /* transform length (the real length for r2c / c2r) */
+    const void *in;   /* device memory, one contiguous line */
+    void *out;        /* device memory, one contiguous line */
+    double fct;
+} rfh_job_t;
+int rfh_batch_create(const rfh_job_t *jobs, int64_t njobs, bool double_precision,
+                     bool autotune, void *stream, void **handle);
+int rfh_batch_execute(void *handle, void *stream);
+/* The same jobs launched one transform at a time through the per-call
+ * engine (the comparison point of the persistent launch). */
+int rfh_batch_execute_serial(void *handle, void *stream);
+int rfh_batch_info(void *handle, int *grid, int *lds_bytes, int *autotuned, double *tuned_ms);
+int rfh_batch_destroy(void *handle);
+
 /* Plan facts of the reference engine, restated bit-exactly:
  * complex FFTPACK factor list (_pocketfft_hdronly.h:1750-1778), real FFTPACK
  * factor list (:2629-2652), and the FFTPACK-vs-Bluestein decision

@@ -46,6 +46,18 @@ __device__ __forceinline__ C2<T> dirw(C2<T> w, T dir) {
   return {w.x, dir < T(0) ? w.y : -w.y};
 }
 
+// Tile index of this workgroup with XCD-contiguous placement.  The
+// dispatcher deals workgroups round-robin to the 8 XCDs (workgroup i runs on
+// XCD i mod 8); renumbering them so that each XCD owns one contiguous run of
+// tiles keeps the cache lines that neighbouring tiles share inside one L2.
+__device__ __forceinline__ unsigned xcd_tile() {
+  constexpr unsigned kXcd = 8;
+  const unsigned g = gridDim.x, i = blockIdx.x;
+  if (g < 2 * kXcd) return i;
+  const unsigned x = i % kXcd, k = i / kXcd, q = g / kXcd, r = g % kXcd;
+  return x * q + (x < r ? x : r) + k;
+}
+
 // q = u / d for 0 <= u < 2^22 via a float reciprocal and one correction.
 __device__ __forceinline__ int fdiv(int u, int d, float inv) {
   int q = int(float(u) * inv);
@@ -302,7 +314,10 @@ enum LineOp {
   OP_R2C_ODD = 5,
   OP_C2R_EVEN = 6,
   OP_C2R_ODD = 7,
+  OP_R2R = 8,     // real line -> pre_elem -> FFT -> post_elem (DCT/DST, Hartley, FFTPACK)
 };
+// OP_R2R packs its modes into LineArgs::sub
+constexpr int R2R_ORTHO = 1 << 16, R2R_CONJ = 1 << 17, R2R_FLIP = 1 << 18;
 enum SubFlags {
   SUB_CHIRP = 1,   // pass A: multiply loads by the chirp, zero beyond n
   SUB_KERNEL = 2,  // pass B: multiply by the Bluestein kernel (natural order)
@@ -367,6 +382,333 @@ template <typename T>
 __device__ __forceinline__ C2<T> big_twiddle(const LineArgs<T> &a, int64_t e) {
   const int64_t mask = (int64_t(1) << a.shift) - 1;
   return cmul(a.t0[e & mask], a.t1[e >> a.shift]);
+}
+
+// e^{i pi num/den}: integer argument reduction, then sincospi in the
+// working precision (FP64 for complex128, FP32 for complex64).
+template <typename T>
+__device__ __forceinline__ C2<T> expipi(int64_t num, int64_t den) {
+  const int64_t per = 2 * den;
+  int64_t r = num % per;
+  if (r < 0) r += per;
+  if constexpr (sizeof(T) == 8) {
+    double s, c;
+    sincospi(double(r) / double(den), &s, &c);
+    return C2<T>{T(c), T(s)};
+  } else {
+    float s, c;
+    sincospif(float(double(r) / double(den)), &s, &c);
+    return C2<T>{c, s};
+  }
+}
+
+// Element j of the complex work line built from input line x (element
+// stride es) for the real-transform modes of PreMode; m = work length.
+template <typename T>
+__device__ __forceinline__ C2<T> pre_elem(int mode, const T *x, int64_t es, int64_t n,
+                                          int64_t m, int64_t j, bool ortho, bool conj,
+                                          bool flip) {
+  const T sqrt2 = T(1.41421356237309504880168872420969808);
+  C2<T> v = {T(0), T(0)};
+  switch (mode) {
+    case kPreRealAsComplex:
+      if (j < n) {
+        T xv = x[j * es];
+        if (flip && j >= 2 && (j % 2) == 0) xv = -xv;
+        v.x = xv;
+      }
+      break;
+    case kPreHermitian: {  // half complex input X[0..n/2]
+      const int64_t h = n / 2;
+      if (j == 0) {
+        v = C2<T>{x[0], T(0)};
+      } else if (j < n - h || (2 * j < n)) {
+        v = C2<T>{x[j * es], x[j * es + 1]};
+        if (conj) v.y = -v.y;
+      } else if (2 * j == n) {
+        v = C2<T>{x[j * es], T(0)};
+      } else {
+        const int64_t k = n - j;
+        v = C2<T>{x[k * es], x[k * es + 1]};
+        if (!conj) v.y = -v.y;
+      }
+      break;
+    }
+    case kPreDct1: {
+      const int64_t src = j < n ? j : m - j;
+      T xv = x[src * es];
+      if (ortho && (src == 0 || src == n - 1)) xv *= sqrt2;
+      v.x = xv;
+      break;
+    }
+    case kPreDst1:
+      if (j >= 1 && j <= n) v.x = x[(j - 1) * es];
+      else if (j >= n + 2) v.x = -x[(m - 1 - j) * es];
+      break;
+    case kPreType2:
+      if (j < n) v.x = x[j * es];
+      break;
+    case kPreDct3:
+      if (j < n) {
+        T xv = x[j * es];
+        if (j == 0 && ortho) xv *= sqrt2;
+        const T w = j == 0 ? T(1) : T(2);
+        v = (w * xv) * expipi<T>(j, 2 * n);
+      }
+      break;
+    case kPreDst3:
+      if (j >= 1 && j <= n) {
+        T xv = x[(j - 1) * es];
+        if (j == 1 && ortho) xv *= sqrt2;
+        const T w = j == n ? T(1) : T(2);
+        v = (w * xv) * expipi<T>(j, 2 * n);
+      }
+      break;
+    case kPreType4:
+      if (j < n) v = x[j * es] * expipi<T>(-j, 2 * n);
+      break;
+    case kPreHalfcomplex: {
+      if (j == 0) {
+        v.x = x[0];
+      } else if (2 * j < n) {
+        v = C2<T>{x[(2 * j - 1) * es], x[(2 * j) * es]};
+        if (conj) v.y = -v.y;
+      } else if (2 * j == n) {
+        v.x = x[(n - 1) * es];
+      } else {
+        const int64_t k = n - j;
+        v = C2<T>{x[(2 * k - 1) * es], -x[(2 * k) * es]};
+        if (conj) v.y = -v.y;
+      }
+      break;
+    }
+  }
+  return v;
+}
+
+// Real output element k from the transformed work line (get(q) = bin q)
+// for the PostMode modes other than kPostComplexHalf; fct not applied.
+template <typename T, typename G>
+__device__ __forceinline__ T post_elem(int mode, G &&get, int64_t n, int64_t k, bool ortho,
+                                       bool flip) {
+  const T isqrt2 = T(0.707106781186547524400844362104849039);
+  T r = T(0);
+  switch (mode) {
+    case kPostReal:
+      r = get(k).x;
+      if (flip && k >= 2 && (k % 2) == 0) r = -r;
+      break;
+    case kPostDct1:
+      r = get(k).x;
+      if (ortho && (k == 0 || k == n - 1)) r *= isqrt2;
+      break;
+    case kPostDst1:
+      r = -get(k + 1).y;
+      break;
+    case kPostDct2: {
+      const C2<T> z = cmul(get(k), expipi<T>(-k, 2 * n));
+      r = T(2) * z.x;
+      if (ortho && k == 0) r *= isqrt2;
+      break;
+    }
+    case kPostDst2: {
+      const C2<T> z = cmul(get(k + 1), expipi<T>(-(k + 1), 2 * n));
+      r = -T(2) * z.y;
+      if (ortho && k == 0) r *= isqrt2;
+      break;
+    }
+    case kPostDct3:
+      r = get(k).x;
+      break;
+    case kPostDst3:
+      r = get(k).y;
+      break;
+    case kPostDct4:
+    case kPostDst4: {
+      const C2<T> z = cmul(get(k), expipi<T>(-(2 * k + 1), 4 * n));
+      r = mode == kPostDct4 ? T(2) * z.x : -T(2) * z.y;
+      break;
+    }
+    case kPostHartley:
+      r = get(k).x + get(k).y;
+      break;
+    case kPostPackHalfcomplex:  // (the forward sign rule flips inputs only)
+      if (k == 0) r = get(0).x;
+      else if (k % 2) r = get((k + 1) / 2).x;
+      else r = get(k / 2).y;
+      break;
+  }
+  return r;
+}
+
+// One tile (a.L lines starting at line tile * a.L) of a line set: load with
+// the op's pre-processing, all Stockham stages out of LDS (two buffers in
+// smem_raw), store with the op's post-processing.  s_oi / s_oo / s_aux are
+// a.L-entry LDS scratch for the per-line offsets.
+template <typename T, int OP, bool VIN, bool VOUT>
+__device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
+                                          unsigned char *smem_raw, int64_t *s_oi,
+                                          int64_t *s_oo, int *s_aux) {
+  C2<T> *bufA = reinterpret_cast<C2<T> *>(smem_raw);
+  C2<T> *bufB = bufA + a.L * a.ld;
+  const int64_t b0 = tile * a.L;
+  const int nl = int(min(int64_t(a.L), a.nlines - b0));
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+    int64_t rem = b0 + l, oi = 0, oo = 0;
+    int aux = 0;
+    for (int d = a.nb - 1; d >= 0; --d) {
+      const int64_t q = rem / a.bdim[d];
+      const int64_t idx = rem - q * a.bdim[d];
+      rem = q;
+      oi += idx * a.bsi[d];
+      oo += idx * a.bso[d];
+      if (d == a.nb - 1) aux = int(idx);
+    }
+    s_oi[l] = oi;
+    s_oo[l] = oo;
+    s_aux[l] = aux;
+  }
+  __syncthreads();
+
+  const int ld = a.ld;
+  const bool conjf = (a.sub & SUB_CONJ) != 0;
+  // ---------------------------------------------------------------- load
+  for_tile(nl, a.nin, a.lfast_in, a.log2L, [&](int l, int j) {
+    const T *base = a.in + s_oi[l];
+    C2<T> v;
+    if constexpr (OP == OP_C2C || OP == OP_PASSB) {
+      v = ldc<VIN>(base + j * a.esi);
+    } else if constexpr (OP == OP_PASSA) {
+      if (a.sub & SUB_CHIRP) {
+        const int64_t m = int64_t(j) * a.other + s_aux[l];
+        if (m < a.bign) {
+          const C2<T> c = a.t2[m];
+          v = cmul(ldc<VIN>(base + j * a.esi), conjf ? c : conjc(c));
+        } else {
+          v = C2<T>{T(0), T(0)};
+        }
+      } else {
+        v = ldc<VIN>(base + j * a.esi);
+      }
+    } else if constexpr (OP == OP_BLUE) {
+      if (j < a.bign) {
+        const C2<T> c = a.t0[j];
+        v = cmul(ldc<VIN>(base + j * a.esi), conjf ? c : conjc(c));
+      } else {
+        v = C2<T>{T(0), T(0)};
+      }
+    } else if constexpr (OP == OP_R2C_EVEN) {
+      if constexpr (VIN)  // contiguous real line: one 2-wide load
+        v = ldc<true>(base + 2 * j);
+      else
+        v = C2<T>{base[2 * j * a.esi], base[(2 * j + 1) * a.esi]};
+    } else if constexpr (OP == OP_R2C_ODD) {
+      v = C2<T>{base[j * a.esi], T(0)};
+    } else if constexpr (OP == OP_C2R_EVEN) {
+      const int h = a.f.n;
+      C2<T> xj = ldc<VIN>(base + j * a.esi), xm = ldc<VIN>(base + (h - j) * a.esi);
+      if (conjf) {
+        xj.y = -xj.y;
+        xm.y = -xm.y;
+      }
+      if (j == 0) {
+        xj.y = T(0);
+        xm.y = T(0);
+      }
+      const C2<T> A = xj + conjc(xm);
+      const C2<T> B = cmul(xj - conjc(xm), conjc(a.t0[j]));
+      v = C2<T>{A.x - B.y, A.y + B.x};
+    } else if constexpr (OP == OP_R2R) {
+      v = pre_elem<T>(a.sub & 0xff, base, a.esi, a.bign, a.f.n, j, (a.sub & R2R_ORTHO) != 0,
+                      (a.sub & R2R_CONJ) != 0, (a.sub & R2R_FLIP) != 0);
+    } else {  // OP_C2R_ODD
+      const int n = a.f.n;
+      if (j == 0) {
+        v = C2<T>{base[0], T(0)};
+      } else if (2 * j < n) {
+        v = ldc<VIN>(base + j * a.esi);
+        if (conjf) v.y = -v.y;
+      } else {
+        v = ldc<VIN>(base + (n - j) * a.esi);
+        if (!conjf) v.y = -v.y;
+      }
+    }
+    bufA[l * ld + j] = v;
+  });
+  __syncthreads();
+
+  // ----------------------------------------------------------------- FFT
+  C2<T> *res = run_fft<T>(a.f, bufA, bufB, a.L, ld, a.tw, a.gen, a.dir);
+  if constexpr (OP == OP_BLUE) {
+    const int n2 = a.f.n;
+    for_tile(a.L, n2, 0, a.log2L, [&](int l, int k) {
+      const C2<T> w = a.t1[k];
+      res[l * ld + k] = cmul(res[l * ld + k], conjf ? conjc(w) : w);
+    });
+    __syncthreads();
+    res = run_fft<T>(a.f, res, res == bufA ? bufB : bufA, a.L, ld, a.tw, a.gen, -a.dir);
+  }
+
+  // --------------------------------------------------------------- store
+  for_tile(nl, a.nout, a.lfast_out, a.log2L, [&](int l, int k) {
+    T *base = a.out + s_oo[l];
+    const C2<T> *r = res + l * ld;
+    if constexpr (OP == OP_C2C) {
+      stc<VOUT>(base + k * a.eso, a.fct * r[k]);
+    } else if constexpr (OP == OP_PASSA) {
+      const C2<T> w = dirw(big_twiddle(a, int64_t(k) * s_aux[l]), a.dir);
+      stc<VOUT>(base + k * a.eso, cmul(r[k], w));
+    } else if constexpr (OP == OP_PASSB) {
+      const int64_t m = s_aux[l] + int64_t(k) * a.other;
+      if (a.sub & SUB_KERNEL) {
+        const C2<T> w = a.t1[m];
+        stc<VOUT>(base + k * a.eso, cmul(r[k], conjf ? conjc(w) : w));
+      } else if (a.sub & SUB_FINAL) {
+        if (m < a.bign) {
+          const C2<T> c = a.t0[m];
+          stc<VOUT>(base + k * a.eso, a.fct * cmul(r[k], conjf ? c : conjc(c)));
+        }
+      } else {
+        stc<VOUT>(base + k * a.eso, a.fct * r[k]);
+      }
+    } else if constexpr (OP == OP_BLUE) {
+      const C2<T> c = a.t0[k];
+      stc<VOUT>(base + k * a.eso, a.fct * cmul(r[k], conjf ? c : conjc(c)));
+    } else if constexpr (OP == OP_R2C_EVEN) {
+      const int h = a.f.n;
+      C2<T> X;
+      if (k == 0 || k == h) {
+        const C2<T> z = r[0];
+        X = C2<T>{k == 0 ? z.x + z.y : z.x - z.y, T(0)};
+      } else {
+        const C2<T> zk = r[k], zm = conjc(r[h - k]);
+        const C2<T> fe = T(0.5) * (zk + zm);
+        const C2<T> dd = zk - zm;
+        const C2<T> fo = C2<T>{T(0.5) * dd.y, T(-0.5) * dd.x};  // (zk - zm) / 2i
+        X = fe + cmul(a.t0[k], fo);
+      }
+      if (conjf) X.y = -X.y;
+      stc<VOUT>(base + k * a.eso, a.fct * X);
+    } else if constexpr (OP == OP_R2C_ODD) {
+      C2<T> X = r[k];
+      if (conjf) X.y = -X.y;
+      stc<VOUT>(base + k * a.eso, a.fct * X);
+    } else if constexpr (OP == OP_C2R_EVEN) {
+      const C2<T> z = a.fct * r[k];
+      if constexpr (VOUT) {  // contiguous real line: one 2-wide store
+        stc<true>(base + 2 * k, z);
+      } else {
+        base[2 * k * a.eso] = z.x;
+        base[(2 * k + 1) * a.eso] = z.y;
+      }
+    } else if constexpr (OP == OP_R2R) {
+      base[k * a.eso] = a.fct * post_elem<T>((a.sub >> 8) & 0xff, [&](int64_t q) { return r[q]; },
+                                             a.bign, k, (a.sub & R2R_ORTHO) != 0,
+                                             (a.sub & R2R_FLIP) != 0);
+    } else {  // OP_C2R_ODD
+      base[k * a.eso] = a.fct * r[k].x;
+    }
+  });
 }
 
 }  // namespace dev

@@ -36,6 +36,11 @@ thread_local hipStream_t t_stream = nullptr;
 thread_local bool t_blocking = true;
 }  // namespace
 
+void thread_stream(hipStream_t *s, bool *blocking) {
+  *s = t_stream;
+  *blocking = t_blocking;
+}
+
 void set_error(int code, const std::string &msg) {
   t_err = code;
   t_msg = msg;
@@ -260,6 +265,11 @@ void r2r_axis(Ctx &c, const int64_t *shape, int nd, int ax, const int64_t *si,
   LineSet g = lines_for_axis(shape, nd, ax, si, so, in, out);
   rr.n = shape[ax];
   rr.m = m;
+  auto plan = get_c2c_plan(uint64_t(m), c.dbl);
+  if (plan->kind == kDirect) {  // the whole work line fits in LDS: one launch
+    launch_r2r_fused(*plan, g, pre, post, rr, fft_forward, fct, c.s);
+    return;
+  }
   void *work = complex_work(c, g.nlines, m);
   launch_pre(pre, rr, g, c.dbl, work, c.s);
   c2c_work(c, work, g.nlines, m, fft_forward);

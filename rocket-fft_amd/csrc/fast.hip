@@ -219,7 +219,8 @@ __global__ void __launch_bounds__(TPL *LPB) k_fast(const LineArgs<T> a) {
   const int t = threadIdx.x;
   const int l = LF_IN ? t % LPB : t / TPL;
   const int i = LF_IN ? t / LPB : t % TPL;
-  const int64_t b = int64_t(blockIdx.x) * LPB + l;
+  const int64_t tile = xcd_tile();
+  const int64_t b = tile * LPB + l;
   FCtx<T> c;
   c.a = &a;
   c.line = lds + l * P;
@@ -251,7 +252,7 @@ __global__ void __launch_bounds__(TPL *LPB) k_fast(const LineArgs<T> a) {
   if constexpr (!DIRECT_OUT) {
     // natural-order result of every line is in LDS; store with the output
     // mapping (the last stage ended with a barrier)
-    const int nl = int(min(int64_t(LPB), a.nlines - int64_t(blockIdx.x) * LPB));
+    const int nl = int(min(int64_t(LPB), a.nlines - tile * LPB));
     const int lo = LF_OUT ? t % LPB : t / TPL;
     const int io = LF_OUT ? t / LPB : t % TPL;
     if (lo < nl) {

@@ -19,7 +19,7 @@
 #include <mutex>
 #include <set>
 
-#include "device.hpp"
+#include "passes.hpp"
 
 namespace rfh {
 namespace {
@@ -30,159 +30,7 @@ __global__ void __launch_bounds__(kThreads) k_lines(const LineArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int64_t s_oi[kMaxLinesPerGroup], s_oo[kMaxLinesPerGroup];
   __shared__ int s_aux[kMaxLinesPerGroup];
-  C2<T> *bufA = reinterpret_cast<C2<T> *>(smem_raw);
-  C2<T> *bufB = bufA + a.L * a.ld;
-  const int64_t b0 = int64_t(blockIdx.x) * a.L;
-  const int nl = int(min(int64_t(a.L), a.nlines - b0));
-  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
-    int64_t rem = b0 + l, oi = 0, oo = 0;
-    int aux = 0;
-    for (int d = a.nb - 1; d >= 0; --d) {
-      const int64_t q = rem / a.bdim[d];
-      const int64_t idx = rem - q * a.bdim[d];
-      rem = q;
-      oi += idx * a.bsi[d];
-      oo += idx * a.bso[d];
-      if (d == a.nb - 1) aux = int(idx);
-    }
-    s_oi[l] = oi;
-    s_oo[l] = oo;
-    s_aux[l] = aux;
-  }
-  __syncthreads();
-
-  const int ld = a.ld;
-  const bool conjf = (a.sub & SUB_CONJ) != 0;
-  // ---------------------------------------------------------------- load
-  for_tile(nl, a.nin, a.lfast_in, a.log2L, [&](int l, int j) {
-    const T *base = a.in + s_oi[l];
-    C2<T> v;
-    if constexpr (OP == OP_C2C || OP == OP_PASSB) {
-      v = ldc<VIN>(base + j * a.esi);
-    } else if constexpr (OP == OP_PASSA) {
-      if (a.sub & SUB_CHIRP) {
-        const int64_t m = int64_t(j) * a.other + s_aux[l];
-        if (m < a.bign) {
-          const C2<T> c = a.t2[m];
-          v = cmul(ldc<VIN>(base + j * a.esi), conjf ? c : conjc(c));
-        } else {
-          v = C2<T>{T(0), T(0)};
-        }
-      } else {
-        v = ldc<VIN>(base + j * a.esi);
-      }
-    } else if constexpr (OP == OP_BLUE) {
-      if (j < a.bign) {
-        const C2<T> c = a.t0[j];
-        v = cmul(ldc<VIN>(base + j * a.esi), conjf ? c : conjc(c));
-      } else {
-        v = C2<T>{T(0), T(0)};
-      }
-    } else if constexpr (OP == OP_R2C_EVEN) {
-      if constexpr (VIN)  // contiguous real line: one 2-wide load
-        v = ldc<true>(base + 2 * j);
-      else
-        v = C2<T>{base[2 * j * a.esi], base[(2 * j + 1) * a.esi]};
-    } else if constexpr (OP == OP_R2C_ODD) {
-      v = C2<T>{base[j * a.esi], T(0)};
-    } else if constexpr (OP == OP_C2R_EVEN) {
-      const int h = a.f.n;
-      C2<T> xj = ldc<VIN>(base + j * a.esi), xm = ldc<VIN>(base + (h - j) * a.esi);
-      if (conjf) {
-        xj.y = -xj.y;
-        xm.y = -xm.y;
-      }
-      if (j == 0) {
-        xj.y = T(0);
-        xm.y = T(0);
-      }
-      const C2<T> A = xj + conjc(xm);
-      const C2<T> B = cmul(xj - conjc(xm), conjc(a.t0[j]));
-      v = C2<T>{A.x - B.y, A.y + B.x};
-    } else {  // OP_C2R_ODD
-      const int n = a.f.n;
-      if (j == 0) {
-        v = C2<T>{base[0], T(0)};
-      } else if (2 * j < n) {
-        v = ldc<VIN>(base + j * a.esi);
-        if (conjf) v.y = -v.y;
-      } else {
-        v = ldc<VIN>(base + (n - j) * a.esi);
-        if (!conjf) v.y = -v.y;
-      }
-    }
-    bufA[l * ld + j] = v;
-  });
-  __syncthreads();
-
-  // ----------------------------------------------------------------- FFT
-  C2<T> *res = run_fft<T>(a.f, bufA, bufB, a.L, ld, a.tw, a.gen, a.dir);
-  if constexpr (OP == OP_BLUE) {
-    const int n2 = a.f.n;
-    for_tile(a.L, n2, 0, a.log2L, [&](int l, int k) {
-      const C2<T> w = a.t1[k];
-      res[l * ld + k] = cmul(res[l * ld + k], conjf ? conjc(w) : w);
-    });
-    __syncthreads();
-    res = run_fft<T>(a.f, res, res == bufA ? bufB : bufA, a.L, ld, a.tw, a.gen, -a.dir);
-  }
-
-  // --------------------------------------------------------------- store
-  for_tile(nl, a.nout, a.lfast_out, a.log2L, [&](int l, int k) {
-    T *base = a.out + s_oo[l];
-    const C2<T> *r = res + l * ld;
-    if constexpr (OP == OP_C2C) {
-      stc<VOUT>(base + k * a.eso, a.fct * r[k]);
-    } else if constexpr (OP == OP_PASSA) {
-      const C2<T> w = dirw(big_twiddle(a, int64_t(k) * s_aux[l]), a.dir);
-      stc<VOUT>(base + k * a.eso, cmul(r[k], w));
-    } else if constexpr (OP == OP_PASSB) {
-      const int64_t m = s_aux[l] + int64_t(k) * a.other;
-      if (a.sub & SUB_KERNEL) {
-        const C2<T> w = a.t1[m];
-        stc<VOUT>(base + k * a.eso, cmul(r[k], conjf ? conjc(w) : w));
-      } else if (a.sub & SUB_FINAL) {
-        if (m < a.bign) {
-          const C2<T> c = a.t0[m];
-          stc<VOUT>(base + k * a.eso, a.fct * cmul(r[k], conjf ? c : conjc(c)));
-        }
-      } else {
-        stc<VOUT>(base + k * a.eso, a.fct * r[k]);
-      }
-    } else if constexpr (OP == OP_BLUE) {
-      const C2<T> c = a.t0[k];
-      stc<VOUT>(base + k * a.eso, a.fct * cmul(r[k], conjf ? c : conjc(c)));
-    } else if constexpr (OP == OP_R2C_EVEN) {
-      const int h = a.f.n;
-      C2<T> X;
-      if (k == 0 || k == h) {
-        const C2<T> z = r[0];
-        X = C2<T>{k == 0 ? z.x + z.y : z.x - z.y, T(0)};
-      } else {
-        const C2<T> zk = r[k], zm = conjc(r[h - k]);
-        const C2<T> fe = T(0.5) * (zk + zm);
-        const C2<T> dd = zk - zm;
-        const C2<T> fo = C2<T>{T(0.5) * dd.y, T(-0.5) * dd.x};  // (zk - zm) / 2i
-        X = fe + cmul(a.t0[k], fo);
-      }
-      if (conjf) X.y = -X.y;
-      stc<VOUT>(base + k * a.eso, a.fct * X);
-    } else if constexpr (OP == OP_R2C_ODD) {
-      C2<T> X = r[k];
-      if (conjf) X.y = -X.y;
-      stc<VOUT>(base + k * a.eso, a.fct * X);
-    } else if constexpr (OP == OP_C2R_EVEN) {
-      const C2<T> z = a.fct * r[k];
-      if constexpr (VOUT) {  // contiguous real line: one 2-wide store
-        stc<true>(base + 2 * k, z);
-      } else {
-        base[2 * k * a.eso] = z.x;
-        base[(2 * k + 1) * a.eso] = z.y;
-      }
-    } else {  // OP_C2R_ODD
-      base[k * a.eso] = a.fct * r[k].x;
-    }
-  });
+  line_body<T, OP, VIN, VOUT>(a, int64_t(xcd_tile()), smem_raw, s_oi, s_oo, s_aux);
 }
 
 // ----------------------------------------------------------- host launch
@@ -272,23 +120,10 @@ void launch_var(const LineArgs<T> &a, size_t lds, bool vin, bool vout, hipStream
   else launch_one<T, OP, false, false>(a, lds, s);
 }
 
-struct PassSpec {
-  const Direct *d;
-  int op;
-  int sub = 0;
-  int dir = -1;
-  double fct = 1.0;
-  int64_t bign = 0, other = 0;
-  int shift = 0;
-  const void *t0 = nullptr, *t1 = nullptr, *t2 = nullptr;
-  int nin = 0, nout = 0;
-  int lines = 0;        // 0 = choose
-  bool real_in = false, real_out = false;
-};
+}  // namespace
 
 template <typename T>
-void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
-  if (g.nlines <= 0) return;
+LineArgs<T> make_line_args(const PassSpec &ps, const LineSet &g) {
   LineArgs<T> a{};
   a.nlines = g.nlines;
   a.nb = g.nb;
@@ -322,6 +157,19 @@ void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
   a.t0 = static_cast<const C2<T> *>(ps.t0);
   a.t1 = static_cast<const C2<T> *>(ps.t1);
   a.t2 = static_cast<const C2<T> *>(ps.t2);
+  return a;
+}
+
+template LineArgs<float> make_line_args<float>(const PassSpec &, const LineSet &);
+template LineArgs<double> make_line_args<double>(const PassSpec &, const LineSet &);
+
+namespace {
+
+template <typename T>
+void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
+  if (g.nlines <= 0) return;
+  LineArgs<T> a = make_line_args<T>(ps, g);
+  const size_t elem = 2 * sizeof(T);
   const size_t lds = size_t(2) * a.L * a.ld * elem;
   if (lds + 8192 > lds_limit()) RFH_THROW(3, "line does not fit in LDS");
   bool vin, vout;
@@ -343,6 +191,7 @@ void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
     case OP_R2C_ODD: launch_var<T, OP_R2C_ODD>(a, lds, false, vout, s); break;
     case OP_C2R_EVEN: launch_var<T, OP_C2R_EVEN>(a, lds, vin, vout, s); break;
     case OP_C2R_ODD: launch_var<T, OP_C2R_ODD>(a, lds, vin, false, s); break;
+    case OP_R2R: launch_one<T, OP_R2R, false, false>(a, lds, s); break;
   }
 }
 
@@ -420,8 +269,9 @@ size_t c2c_tmp_bytes(const C2CPlan &p, int64_t nlines) {
   return 0;
 }
 
-void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStream_t s,
-                void *tmp) {
+std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct,
+                                     void *tmp) {
+  std::vector<PassLaunch> out;
   const bool dbl = p.dbl;
   const int sub_conj = dir > 0 ? SUB_CONJ : 0;
   if (p.kind == kDirect) {
@@ -431,7 +281,7 @@ void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStre
     ps.dir = dir;
     ps.fct = fct;
     ps.nin = ps.nout = int(p.n);
-    run_pass_any(dbl, ps, g, s);
+    out.push_back(PassLaunch{ps, g});
   } else if (p.kind == kBlueDirect) {
     PassSpec ps;
     ps.d = &p.main;
@@ -444,7 +294,7 @@ void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStre
     ps.t1 = p.bkf.p;
     ps.nin = int(p.bn2);
     ps.nout = int(p.n);
-    run_pass_any(dbl, ps, g, s);
+    out.push_back(PassLaunch{ps, g});
   } else {
     const uint64_t n1 = p.n1, n2 = p.n2, N = n1 * n2;
     const int lines = dbl ? 4 : 8;
@@ -459,7 +309,7 @@ void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStre
       a.t1 = p.big_hi.p;
       a.nin = a.nout = int(n1);
       a.lines = lines;
-      run_pass_any(dbl, a, pass_a_lines(g, n1, n2, g.in, g.esi, tmp), s);
+      out.push_back(PassLaunch{a, pass_a_lines(g, n1, n2, g.in, g.esi, tmp)});
       PassSpec b;
       b.d = &p.pb;
       b.op = OP_PASSB;
@@ -468,7 +318,7 @@ void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStre
       b.other = int64_t(n1);
       b.nin = b.nout = int(n2);
       b.lines = lines;
-      run_pass_any(dbl, b, pass_b_lines(g, n1, n2, tmp, g.out, g.eso, g.bso), s);
+      out.push_back(PassLaunch{b, pass_b_lines(g, n1, n2, tmp, g.out, g.eso, g.bso)});
     } else {  // Bluestein in four passes through two temporaries
       char *t1 = static_cast<char *>(tmp);
       char *t2 = t1 + size_t(g.nlines) * N * (dbl ? 16 : 8);
@@ -492,7 +342,7 @@ void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStre
       a.t2 = p.bk.p;
       a.nin = a.nout = int(n1);
       a.lines = lines;
-      run_pass_any(dbl, a, pass_a_lines(g, n1, n2, g.in, g.esi, t1), s);
+      out.push_back(PassLaunch{a, pass_a_lines(g, n1, n2, g.in, g.esi, t1)});
       PassSpec b;  // forward rows, times the kernel, natural order into t2
       b.d = &p.pb;
       b.op = OP_PASSB;
@@ -502,12 +352,12 @@ void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStre
       b.t1 = p.bkf.p;
       b.nin = b.nout = int(n2);
       b.lines = lines;
-      run_pass_any(dbl, b, pass_b_lines(gc, n1, n2, t1, t2, 2, gc.bso), s);
+      out.push_back(PassLaunch{b, pass_b_lines(gc, n1, n2, t1, t2, 2, gc.bso)});
       PassSpec a3 = a;  // backward columns
       a3.dir = +1;
       a3.sub = 0;
       a3.t2 = nullptr;
-      run_pass_any(dbl, a3, pass_a_lines(gc, n1, n2, t2, 2, t1), s);
+      out.push_back(PassLaunch{a3, pass_a_lines(gc, n1, n2, t2, 2, t1)});
       PassSpec b4;  // backward rows, times chirp and fct, first n outputs
       b4.d = &p.pb;
       b4.op = OP_PASSB;
@@ -519,13 +369,34 @@ void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStre
       b4.t0 = p.bk.p;
       b4.nin = b4.nout = int(n2);
       b4.lines = lines;
-      run_pass_any(dbl, b4, pass_b_lines(g, n1, n2, t1, g.out, g.eso, g.bso), s);
+      out.push_back(PassLaunch{b4, pass_b_lines(g, n1, n2, t1, g.out, g.eso, g.bso)});
     }
   }
+  return out;
 }
 
-void launch_real_fused(const RealPlan &rp, RealOp op, const LineSet &g, bool conj_flag,
-                       double fct, hipStream_t s) {
+void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStream_t s,
+                void *tmp) {
+  for (const PassLaunch &pl : describe_c2c(p, g, dir, fct, tmp)) run_pass_any(p.dbl, pl.ps, pl.g, s);
+}
+
+void launch_r2r_fused(const C2CPlan &p, const LineSet &g, PreMode pre, PostMode post,
+                      const RealPrePost &rr, bool fft_forward, double fct, hipStream_t s) {
+  PassSpec ps;
+  ps.d = &p.main;
+  ps.op = OP_R2R;
+  ps.dir = fft_forward ? -1 : +1;
+  ps.fct = fct;
+  ps.bign = rr.n;
+  ps.sub = int(pre) | (int(post) << 8) | (rr.ortho ? R2R_ORTHO : 0) |
+           (rr.conj ? R2R_CONJ : 0) | (rr.flip ? R2R_FLIP : 0);
+  ps.nin = int(rr.m);
+  ps.nout = int(rr.n);
+  run_pass_any(p.dbl, ps, g, s);
+}
+
+PassLaunch describe_real_fused(const RealPlan &rp, RealOp op, const LineSet &g, bool conj_flag,
+                               double fct) {
   PassSpec ps;
   ps.d = &rp.d;
   ps.fct = fct;
@@ -546,7 +417,13 @@ void launch_real_fused(const RealPlan &rp, RealOp op, const LineSet &g, bool con
     ps.nout = int(even ? n / 2 : n);
     ps.real_out = true;
   }
-  run_pass_any(rp.dbl, ps, g, s);
+  return PassLaunch{ps, g};
+}
+
+void launch_real_fused(const RealPlan &rp, RealOp op, const LineSet &g, bool conj_flag,
+                       double fct, hipStream_t s) {
+  const PassLaunch pl = describe_real_fused(rp, op, g, conj_flag, fct);
+  run_pass_any(rp.dbl, pl.ps, pl.g, s);
 }
 
 // ------------------------------------------------------- pre / post kernels
@@ -568,17 +445,6 @@ struct PrePostArgs {
 };
 
 template <typename T>
-__device__ __forceinline__ C2<T> expipi(int64_t num, int64_t den) {
-  // e^{i pi num/den}, argument reduced in integers first
-  const int64_t per = 2 * den;
-  int64_t r = num % per;
-  if (r < 0) r += per;
-  double s, c;
-  sincospi(double(r) / double(den), &s, &c);
-  return C2<T>{T(c), T(s)};
-}
-
-template <typename T>
 __device__ __forceinline__ int64_t line_offset(const PrePostArgs<T> &a, int64_t b) {
   int64_t rem = b, o = 0;
   for (int d = a.nb - 1; d >= 0; --d) {
@@ -591,154 +457,33 @@ __device__ __forceinline__ int64_t line_offset(const PrePostArgs<T> &a, int64_t 
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_pre(const PrePostArgs<T> a) {
-  const T sqrt2 = T(1.41421356237309504880168872420969808);
   for (int64_t b = blockIdx.y; b < a.nlines; b += gridDim.y) {
     const T *x = a.rin + line_offset(a, b);
     C2<T> *t = a.tmp + b * a.m;
     const int64_t n = a.n;
     for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < a.m;
          j += int64_t(gridDim.x) * blockDim.x) {
-      C2<T> v = {T(0), T(0)};
-      switch (a.mode) {
-        case kPreRealAsComplex:
-          if (j < n) {
-            T xv = x[j * a.es];
-            if (a.flip && j >= 2 && (j % 2) == 0) xv = -xv;
-            v.x = xv;
-          }
-          break;
-        case kPreHermitian: {  // half complex input X[0..n/2]
-          const int64_t h = n / 2;
-          if (j == 0) {
-            v = C2<T>{x[0], T(0)};
-          } else if (j < n - h || (2 * j < n)) {
-            v = C2<T>{x[j * a.es], x[j * a.es + 1]};
-            if (a.conj) v.y = -v.y;
-          } else if (2 * j == n) {
-            v = C2<T>{x[j * a.es], T(0)};
-          } else {
-            const int64_t k = n - j;
-            v = C2<T>{x[k * a.es], x[k * a.es + 1]};
-            if (!a.conj) v.y = -v.y;
-          }
-          break;
-        }
-        case kPreDct1: {
-          const int64_t src = j < n ? j : a.m - j;
-          T xv = x[src * a.es];
-          if (a.ortho && (src == 0 || src == n - 1)) xv *= sqrt2;
-          v.x = xv;
-          break;
-        }
-        case kPreDst1:
-          if (j >= 1 && j <= n) v.x = x[(j - 1) * a.es];
-          else if (j >= n + 2) v.x = -x[(a.m - 1 - j) * a.es];
-          break;
-        case kPreType2:
-          if (j < n) v.x = x[j * a.es];
-          break;
-        case kPreDct3:
-          if (j < n) {
-            T xv = x[j * a.es];
-            if (j == 0 && a.ortho) xv *= sqrt2;
-            const T w = j == 0 ? T(1) : T(2);
-            v = (w * xv) * expipi<T>(j, 2 * n);
-          }
-          break;
-        case kPreDst3:
-          if (j >= 1 && j <= n) {
-            T xv = x[(j - 1) * a.es];
-            if (j == 1 && a.ortho) xv *= sqrt2;
-            const T w = j == n ? T(1) : T(2);
-            v = (w * xv) * expipi<T>(j, 2 * n);
-          }
-          break;
-        case kPreType4:
-          if (j < n) v = x[j * a.es] * expipi<T>(-j, 2 * n);
-          break;
-        case kPreHalfcomplex: {
-          if (j == 0) {
-            v.x = x[0];
-          } else if (2 * j < n) {
-            v = C2<T>{x[(2 * j - 1) * a.es], x[(2 * j) * a.es]};
-            if (a.conj) v.y = -v.y;
-          } else if (2 * j == n) {
-            v.x = x[(n - 1) * a.es];
-          } else {
-            const int64_t k = n - j;
-            v = C2<T>{x[(2 * k - 1) * a.es], -x[(2 * k) * a.es]};
-            if (a.conj) v.y = -v.y;
-          }
-          break;
-        }
-      }
-      t[j] = v;
+      t[j] = pre_elem<T>(a.mode, x, a.es, n, a.m, j, a.ortho, a.conj, a.flip);
     }
   }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_post(const PrePostArgs<T> a) {
-  const T isqrt2 = T(0.707106781186547524400844362104849039);
   for (int64_t b = blockIdx.y; b < a.nlines; b += gridDim.y) {
     T *y = a.rout + line_offset(a, b);
     const C2<T> *t = a.tmp + b * a.m;
     const int64_t n = a.n;
     for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < a.count;
          k += int64_t(gridDim.x) * blockDim.x) {
-      T r = T(0);
-      switch (a.mode) {
-        case kPostComplexHalf: {
-          C2<T> v = t[k];
-          if (a.conj) v.y = -v.y;
-          y[k * a.es] = a.fct * v.x;
-          y[k * a.es + 1] = a.fct * v.y;
-          continue;
-        }
-        case kPostReal:
-          r = t[k].x;
-          if (a.flip && k >= 2 && (k % 2) == 0) r = -r;
-          break;
-        case kPostDct1:
-          r = t[k].x;
-          if (a.ortho && (k == 0 || k == n - 1)) r *= isqrt2;
-          break;
-        case kPostDst1:
-          r = -t[k + 1].y;
-          break;
-        case kPostDct2: {
-          const C2<T> z = cmul(t[k], expipi<T>(-k, 2 * n));
-          r = T(2) * z.x;
-          if (a.ortho && k == 0) r *= isqrt2;
-          break;
-        }
-        case kPostDst2: {
-          const C2<T> z = cmul(t[k + 1], expipi<T>(-(k + 1), 2 * n));
-          r = -T(2) * z.y;
-          if (a.ortho && k == 0) r *= isqrt2;
-          break;
-        }
-        case kPostDct3:
-          r = t[k].x;
-          break;
-        case kPostDst3:
-          r = t[k].y;
-          break;
-        case kPostDct4:
-        case kPostDst4: {
-          const C2<T> z = cmul(t[k], expipi<T>(-(2 * k + 1), 4 * n));
-          r = a.mode == kPostDct4 ? T(2) * z.x : -T(2) * z.y;
-          break;
-        }
-        case kPostHartley:
-          r = t[k].x + t[k].y;
-          break;
-        case kPostPackHalfcomplex:  // (the forward sign rule flips inputs only)
-          if (k == 0) r = t[0].x;
-          else if (k % 2) r = t[(k + 1) / 2].x;
-          else r = t[k / 2].y;
-          break;
+      if (a.mode == kPostComplexHalf) {
+        C2<T> v = t[k];
+        if (a.conj) v.y = -v.y;
+        y[k * a.es] = a.fct * v.x;
+        y[k * a.es + 1] = a.fct * v.y;
+        continue;
       }
+      const T r = post_elem<T>(a.mode, [&](int64_t q) { return t[q]; }, n, k, a.ortho, a.flip);
       y[k * a.es] = a.fct * r;
     }
   }

@@ -224,9 +224,9 @@ size_t lds_limit() {
 // Static LDS of the line kernels (per-line offsets) is reserved first.
 static size_t lds_dynamic_budget() { return lds_limit() - 8192; }
 
-uint64_t direct_max(bool dbl, int lines) {
+uint64_t direct_max(bool dbl, int lines, size_t budget) {
   const size_t elem = dbl ? 16 : 8;
-  return lds_dynamic_budget() / (2 * elem * size_t(lines));
+  return (budget ? budget : lds_dynamic_budget()) / (2 * elem * size_t(lines));
 }
 
 // --------------------------------------------------------------- twiddles
@@ -355,8 +355,8 @@ static void make_bluestein(C2CPlan &p, uint64_t n, uint64_t n2) {
 
 // Two-pass split N = n1 * n2 with both passes in LDS at `lines` lines per
 // group; the most balanced split wins.  0 when none exists.
-static uint64_t pick_split(uint64_t N, bool dbl, int lines) {
-  const uint64_t lim = direct_max(dbl, lines);
+static uint64_t pick_split(uint64_t N, bool dbl, int lines, size_t budget) {
+  const uint64_t lim = direct_max(dbl, lines, budget);
   uint64_t best = 0;
   double bestd = 1e300;
   for (uint64_t d = 2; d <= lim && d <= N / 2; ++d) {
@@ -374,8 +374,8 @@ static uint64_t pick_split(uint64_t N, bool dbl, int lines) {
 
 static int two_pass_lines(bool dbl) { return dbl ? 4 : 8; }  // 64-byte row segments
 
-static void make_two_pass(C2CPlan &p, uint64_t N) {
-  const uint64_t n1 = pick_split(N, p.dbl, two_pass_lines(p.dbl));
+static void make_two_pass(C2CPlan &p, uint64_t N, size_t budget) {
+  const uint64_t n1 = pick_split(N, p.dbl, two_pass_lines(p.dbl), budget);
   if (!n1) RFH_THROW(3, "no two-pass split for length " + std::to_string(N));
   p.n1 = n1;
   p.n2 = N / n1;
@@ -392,29 +392,29 @@ static void make_two_pass(C2CPlan &p, uint64_t N) {
   upload_any(p.big_hi, hi, p.dbl);
 }
 
-static std::shared_ptr<C2CPlan> build_c2c(uint64_t n, bool dbl) {
+static std::shared_ptr<C2CPlan> build_c2c(uint64_t n, bool dbl, size_t budget) {
   auto p = std::make_shared<C2CPlan>();
   p->n = n;
   p->dbl = dbl;
   uint64_t bn2 = 0;
   bool blue = n > 1 && plan_bluestein(n, false, &bn2);
   const uint64_t N = blue ? bn2 : n;
-  if (N <= direct_max(dbl, 1)) {
+  if (N <= direct_max(dbl, 1, budget)) {
     p->kind = blue ? kBlueDirect : kDirect;
     make_direct(p->main, N, dbl);
-  } else if (pick_split(N, dbl, two_pass_lines(dbl))) {
+  } else if (pick_split(N, dbl, two_pass_lines(dbl), budget)) {
     p->kind = blue ? kBlueTwoPass : kTwoPass;
-    make_two_pass(*p, N);
+    make_two_pass(*p, N, budget);
   } else if (!blue) {
     // a long length the two-pass split cannot factor: Bluestein always can
     blue = true;
     bn2 = good_size_cmplx(2 * n - 1);
-    if (bn2 <= direct_max(dbl, 1)) {
+    if (bn2 <= direct_max(dbl, 1, budget)) {
       p->kind = kBlueDirect;
       make_direct(p->main, bn2, dbl);
     } else {
       p->kind = kBlueTwoPass;
-      make_two_pass(*p, bn2);
+      make_two_pass(*p, bn2, budget);
     }
   } else {
     RFH_THROW(3, "unsupported transform length " + std::to_string(n));
@@ -423,7 +423,7 @@ static std::shared_ptr<C2CPlan> build_c2c(uint64_t n, bool dbl) {
   return p;
 }
 
-static std::shared_ptr<RealPlan> build_real(uint64_t n, bool dbl) {
+static std::shared_ptr<RealPlan> build_real(uint64_t n, bool dbl, size_t budget) {
   auto p = std::make_shared<RealPlan>();
   p->n = n;
   p->dbl = dbl;
@@ -431,7 +431,7 @@ static std::shared_ptr<RealPlan> build_real(uint64_t n, bool dbl) {
   uint64_t bn2;
   // odd lengths that the reference sends to Bluestein stay on the generic path
   const bool blue = (n % 2 == 1) && n > 1 && plan_bluestein(n, false, &bn2);
-  if (n >= 2 && !blue && m <= direct_max(dbl, 1)) {
+  if (n >= 2 && !blue && m <= direct_max(dbl, 1, budget)) {
     p->fused = true;
     make_direct(p->d, m, dbl);
     if (n % 2 == 0) {
@@ -447,42 +447,35 @@ static std::shared_ptr<RealPlan> build_real(uint64_t n, bool dbl) {
 
 namespace {
 std::mutex g_cache_mu;
-std::map<std::tuple<int, uint64_t, bool>, std::shared_ptr<C2CPlan>> g_c2c;
-std::map<std::tuple<int, uint64_t, bool>, std::shared_ptr<RealPlan>> g_real;
-}  // namespace
+std::map<std::tuple<int, uint64_t, bool, size_t>, std::shared_ptr<C2CPlan>> g_c2c;
+std::map<std::tuple<int, uint64_t, bool, size_t>, std::shared_ptr<RealPlan>> g_real;
 
-std::shared_ptr<C2CPlan> get_c2c_plan(uint64_t n, bool dbl) {
+template <typename P, typename B>
+std::shared_ptr<P> cached(std::map<std::tuple<int, uint64_t, bool, size_t>, std::shared_ptr<P>> &m,
+                          uint64_t n, bool dbl, size_t budget, B &&build) {
   int dev = 0;
   HIP_CHECK(hipGetDevice(&dev));
-  auto key = std::make_tuple(dev, n, dbl);
+  auto key = std::make_tuple(dev, n, dbl, budget);
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);
-    auto it = g_c2c.find(key);
-    if (it != g_c2c.end()) return it->second;
+    auto it = m.find(key);
+    if (it != m.end()) return it->second;
   }
-  auto p = build_c2c(n, dbl);  // built outside the lock, re-checked below
+  auto p = build(n, dbl, budget ? budget : lds_dynamic_budget());  // outside the lock
   std::lock_guard<std::mutex> lk(g_cache_mu);
-  auto it = g_c2c.find(key);
-  if (it != g_c2c.end()) return it->second;
-  g_c2c[key] = p;
+  auto it = m.find(key);
+  if (it != m.end()) return it->second;
+  m[key] = p;
   return p;
 }
+}  // namespace
 
-std::shared_ptr<RealPlan> get_real_plan(uint64_t n, bool dbl) {
-  int dev = 0;
-  HIP_CHECK(hipGetDevice(&dev));
-  auto key = std::make_tuple(dev, n, dbl);
-  {
-    std::lock_guard<std::mutex> lk(g_cache_mu);
-    auto it = g_real.find(key);
-    if (it != g_real.end()) return it->second;
-  }
-  auto p = build_real(n, dbl);
-  std::lock_guard<std::mutex> lk(g_cache_mu);
-  auto it = g_real.find(key);
-  if (it != g_real.end()) return it->second;
-  g_real[key] = p;
-  return p;
+std::shared_ptr<C2CPlan> get_c2c_plan(uint64_t n, bool dbl, size_t budget) {
+  return cached(g_c2c, n, dbl, budget, build_c2c);
+}
+
+std::shared_ptr<RealPlan> get_real_plan(uint64_t n, bool dbl, size_t budget) {
+  return cached(g_real, n, dbl, budget, build_real);
 }
 
 }  // namespace rfh

@@ -1,0 +1,512 @@
+// ragged.hip -- many independent transforms of uneven lengths and kinds
+// (complex c2c, real r2c / c2r) in one persistent launch.
+//
+// A batch is planned once (rfh_batch_create): every job gets the passes the
+// per-call engine would run for it (direct LDS line, Bluestein in LDS, the
+// four-step passes, or pre -> c2c -> post for real lengths without a fused
+// plan), all sized to one LDS budget, with its temporaries carved out of one
+// arena.  Execution is two kernels: a fill that zeroes the job queue and one
+// persistent kernel whose workgroups pull whole jobs from that queue (largest
+// first) and run every pass of their job themselves, tile after tile, with
+// workgroup barriers between passes.  No workgroup ever waits on another,
+// so the launch always drains; each job's data stays on one CU and its
+// XCD's L2.  The LDS budget and the grid size are picked by a small
+// autotuner that times the candidates on the batch itself.
+//
+// Execution allocates nothing and calls no host synchronisation, so it can
+// be captured in a HIP graph.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+#include "../../include/rocket_fft_hip.h"
+#include "passes.hpp"
+
+namespace rfh {
+namespace {
+using namespace dev;
+
+constexpr int OP_PRE = 9;   // one line: real / half-complex input -> complex work line
+constexpr int OP_POST = 10; // one line: complex work line -> real / half-complex output
+
+struct JobDev {
+  int first;  // first pass
+  int npass;
+};
+struct PassDev {
+  int op;
+  int tiles;
+};
+
+template <typename T>
+__device__ void pre_line(const LineArgs<T> &a) {
+  C2<T> *t = reinterpret_cast<C2<T> *>(a.out);
+  const int mode = a.sub & 0xff;
+  for (int64_t j = threadIdx.x; j < a.f.n; j += blockDim.x)
+    t[j] = pre_elem<T>(mode, a.in, a.esi, a.bign, a.f.n, j, false, (a.sub & R2R_CONJ) != 0,
+                       false);
+}
+
+template <typename T>
+__device__ void post_line(const LineArgs<T> &a) {
+  const C2<T> *t = reinterpret_cast<const C2<T> *>(a.in);
+  const int mode = (a.sub >> 8) & 0xff;
+  const int64_t n = a.bign;
+  if (mode == kPostComplexHalf) {
+    for (int64_t k = threadIdx.x; k <= n / 2; k += blockDim.x) {
+      C2<T> v = t[k];
+      if (a.sub & R2R_CONJ) v.y = -v.y;
+      a.out[k * a.eso] = a.fct * v.x;
+      a.out[k * a.eso + 1] = a.fct * v.y;
+    }
+  } else {
+    for (int64_t k = threadIdx.x; k < n; k += blockDim.x)
+      a.out[k * a.eso] =
+          a.fct * post_elem<T>(mode, [&](int64_t q) { return t[q]; }, n, k, false, false);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_ragged(const JobDev *__restrict__ jobs,
+                                                    const int *__restrict__ order, int njobs,
+                                                    const PassDev *__restrict__ passes,
+                                                    const LineArgs<T> *__restrict__ args,
+                                                    unsigned *queue) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ int64_t s_oi[kMaxLinesPerGroup], s_oo[kMaxLinesPerGroup];
+  __shared__ int s_aux[kMaxLinesPerGroup];
+  __shared__ int s_job;
+  for (;;) {
+    if (threadIdx.x == 0) s_job = int(atomicAdd(queue, 1u));
+    __syncthreads();
+    const int q = __builtin_amdgcn_readfirstlane(s_job);
+    __syncthreads();  // every lane has read s_job before the next grab
+    if (q >= njobs) break;
+    const JobDev jb = jobs[order[q]];
+    for (int p = 0; p < jb.npass; ++p) {
+      const PassDev pd = passes[jb.first + p];
+      const LineArgs<T> &a = args[jb.first + p];
+      for (int t = 0; t < pd.tiles; ++t) {
+        switch (pd.op) {
+          case OP_C2C: line_body<T, OP_C2C, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+          case OP_PASSA: line_body<T, OP_PASSA, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+          case OP_PASSB: line_body<T, OP_PASSB, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+          case OP_BLUE: line_body<T, OP_BLUE, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+          case OP_R2C_EVEN: line_body<T, OP_R2C_EVEN, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+          case OP_R2C_ODD: line_body<T, OP_R2C_ODD, false, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+          case OP_C2R_EVEN: line_body<T, OP_C2R_EVEN, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+          case OP_C2R_ODD: line_body<T, OP_C2R_ODD, true, false>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+          case OP_PRE: pre_line<T>(a); break;
+          default: post_line<T>(a); break;
+        }
+        __syncthreads();  // pass and tile boundary: global writes visible to the workgroup
+      }
+    }
+  }
+}
+
+__global__ void k_zero(unsigned *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+// ------------------------------------------------------------------ host
+
+struct Built {
+  size_t budget = 0;          // dynamic LDS per workgroup
+  DevBuf jobs, order, passes, args, queue, arena;
+  std::vector<std::shared_ptr<C2CPlan>> c2c_keep;
+  std::vector<std::shared_ptr<RealPlan>> real_keep;
+  int njobs = 0;
+  int lds_need = 0;
+};
+
+struct Batch {
+  bool dbl = false;
+  std::vector<rfh_job_t> jobs;
+  std::unique_ptr<Built> b;
+  int grid = 0;
+  int autotuned = 0;
+  double tuned_ms = 0;
+};
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+LineSet one_line(const void *in, void *out, int64_t esi, int64_t eso) {
+  LineSet g{};
+  g.in = in;
+  g.out = out;
+  g.nlines = 1;
+  g.nb = 0;
+  g.esi = esi;
+  g.eso = eso;
+  return g;
+}
+
+// cost estimate for largest-first ordering: bytes moved by all passes
+double job_cost(const std::vector<PassLaunch> &pl) {
+  double c = 0;
+  for (const auto &p : pl) c += double(p.g.nlines) * double(p.ps.nin + p.ps.nout);
+  return c;
+}
+
+template <typename T>
+std::unique_ptr<Built> build(const std::vector<rfh_job_t> &jobs, bool dbl, size_t budget) {
+  auto B = std::make_unique<Built>();
+  B->budget = budget;
+  B->njobs = int(jobs.size());
+  const size_t ce = 2 * sizeof(T);
+  // pass 1: plans and temporaries
+  std::vector<std::vector<PassLaunch>> per(jobs.size());
+  std::vector<size_t> tmp_off(jobs.size()), work_off(jobs.size());
+  size_t arena = 0;
+  std::vector<std::shared_ptr<C2CPlan>> cplan(jobs.size());
+  std::vector<std::shared_ptr<RealPlan>> rplan(jobs.size());
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    const rfh_job_t &j = jobs[i];
+    if (j.n == 0) continue;
+    if (j.kind == RFH_JOB_C2C) {
+      cplan[i] = get_c2c_plan(j.n, dbl, budget);
+      tmp_off[i] = arena;
+      arena = align_up(arena + c2c_tmp_bytes(*cplan[i], 1), 256);
+    } else {
+      rplan[i] = get_real_plan(j.n, dbl, budget);
+      if (!rplan[i]->fused) {  // pre -> c2c(n) -> post through a work line
+        cplan[i] = get_c2c_plan(j.n, dbl, budget);
+        work_off[i] = arena;
+        arena = align_up(arena + size_t(j.n) * ce, 256);
+        tmp_off[i] = arena;
+        arena = align_up(arena + c2c_tmp_bytes(*cplan[i], 1), 256);
+      }
+    }
+  }
+  B->arena.bytes = 0;
+  if (arena) {
+    HIP_CHECK(hipMalloc(&B->arena.p, arena));
+    B->arena.bytes = arena;
+  }
+  char *ar = static_cast<char *>(B->arena.p);
+  // pass 2: the passes of every job
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    const rfh_job_t &j = jobs[i];
+    if (j.n == 0) continue;
+    const int dir = j.forward ? -1 : +1;
+    if (j.kind == RFH_JOB_C2C) {
+      per[i] = describe_c2c(*cplan[i], one_line(j.in, j.out, 2, 2), dir, j.fct, ar + tmp_off[i]);
+      continue;
+    }
+    const bool r2c = j.kind == RFH_JOB_R2C;
+    if (rplan[i]->fused) {
+      per[i].push_back(describe_real_fused(*rplan[i], r2c ? kR2C : kC2R,
+                                           one_line(j.in, j.out, r2c ? 1 : 2, r2c ? 2 : 1),
+                                           r2c ? !j.forward : j.forward, j.fct));
+      continue;
+    }
+    void *work = ar + work_off[i];
+    PassSpec pre{};
+    pre.d = &cplan[i]->main;  // only desc.n (= m) is read
+    pre.op = OP_PRE;
+    pre.bign = int64_t(j.n);
+    pre.sub = int(r2c ? kPreRealAsComplex : kPreHermitian) | (!r2c && j.forward ? R2R_CONJ : 0);
+    per[i].push_back(PassLaunch{pre, one_line(j.in, work, r2c ? 1 : 2, 2)});
+    for (auto &pl : describe_c2c(*cplan[i], one_line(work, work, 2, 2), r2c ? -1 : +1, 1.0,
+                                 ar + tmp_off[i]))
+      per[i].push_back(pl);
+    PassSpec post{};
+    post.d = &cplan[i]->main;
+    post.op = OP_POST;
+    post.bign = int64_t(j.n);
+    post.fct = j.fct;
+    post.sub = (int(r2c ? kPostComplexHalf : kPostReal) << 8) | (r2c && !j.forward ? R2R_CONJ : 0);
+    per[i].push_back(PassLaunch{post, one_line(work, j.out, 2, r2c ? 2 : 1)});
+  }
+  // pass 3: device tables
+  std::vector<JobDev> jd(jobs.size());
+  std::vector<PassDev> pd;
+  std::vector<LineArgs<T>> la;
+  std::vector<std::pair<double, int>> cost;
+  int lds_need = 0;
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    jd[i].first = int(pd.size());
+    jd[i].npass = int(per[i].size());
+    for (const PassLaunch &pl : per[i]) {
+      LineArgs<T> a = make_line_args<T>(pl.ps, pl.g);
+      if (pl.ps.op == OP_PRE || pl.ps.op == OP_POST) {
+        a.f.n = int(cplan[i]->n);  // work-line length m (= n)
+        a.L = 1;
+      } else {
+        lds_need = std::max<int>(lds_need, int(2 * a.L * a.ld * ce));
+      }
+      pd.push_back(PassDev{pl.ps.op, int((pl.g.nlines + a.L - 1) / a.L)});
+      la.push_back(a);
+    }
+    cost.push_back({job_cost(per[i]), int(i)});
+  }
+  if (size_t(lds_need) > budget) RFH_THROW(3, "ragged batch: a pass exceeds the LDS budget");
+  B->lds_need = lds_need;
+  std::stable_sort(cost.begin(), cost.end(),
+                   [](const std::pair<double, int> &x, const std::pair<double, int> &y) {
+                     return x.first > y.first;
+                   });
+  std::vector<int> order;
+  for (auto &c : cost) order.push_back(c.second);
+  B->jobs.upload(jd.data(), jd.size() * sizeof(JobDev));
+  B->order.upload(order.data(), order.size() * sizeof(int));
+  B->passes.upload(pd.data(), pd.size() * sizeof(PassDev));
+  B->args.upload(la.data(), la.size() * sizeof(LineArgs<T>));
+  unsigned zero = 0;
+  B->queue.upload(&zero, sizeof(zero));
+  for (auto &p : cplan)
+    if (p) B->c2c_keep.push_back(p);
+  for (auto &p : rplan)
+    if (p) B->real_keep.push_back(p);
+  return B;
+}
+
+std::mutex g_attr_mu;
+bool g_attr_set[2] = {false, false};
+
+template <typename T>
+void launch_batch(const Built &B, int grid, hipStream_t s) {
+  if (B.njobs == 0) return;
+  const void *fn = reinterpret_cast<const void *>(&k_ragged<T>);
+  {
+    std::lock_guard<std::mutex> lk(g_attr_mu);
+    if (!g_attr_set[sizeof(T) == 8]) {
+      hipFuncAttributes fa;
+      HIP_CHECK(hipFuncGetAttributes(&fa, fn));
+      HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(lds_limit() - fa.sharedSizeBytes)));
+      g_attr_set[sizeof(T) == 8] = true;
+    }
+  }
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, static_cast<unsigned *>(B.queue.p), 1);
+  hipLaunchKernelGGL(k_ragged<T>, dim3(unsigned(grid)), dim3(kThreads), size_t(B.lds_need), s,
+                     static_cast<const JobDev *>(B.jobs.p), static_cast<const int *>(B.order.p),
+                     B.njobs, static_cast<const PassDev *>(B.passes.p),
+                     static_cast<const LineArgs<T> *>(B.args.p),
+                     static_cast<unsigned *>(B.queue.p));
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) RFH_THROW(2, std::string("ragged launch: ") + hipGetErrorString(e));
+}
+
+int cu_count() {
+  static int n = [] {
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      v = 256;
+    return v > 0 ? v : 256;
+  }();
+  return n;
+}
+
+void run(Batch &bt, int grid, hipStream_t s) {
+  if (bt.dbl)
+    launch_batch<double>(*bt.b, grid, s);
+  else
+    launch_batch<float>(*bt.b, grid, s);
+}
+
+double time_ms(Batch &bt, int grid, hipStream_t s, int reps) {
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  run(bt, grid, s);  // warm
+  HIP_CHECK(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) run(bt, grid, s);
+  HIP_CHECK(hipEventRecord(e1, s));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return double(ms) / reps;
+}
+
+std::unique_ptr<Built> build_any(const Batch &bt, size_t budget) {
+  return bt.dbl ? build<double>(bt.jobs, true, budget) : build<float>(bt.jobs, false, budget);
+}
+
+// Autotuner: LDS budget (which sets the four-step splits and how many
+// workgroups share a CU) x grid size; each candidate is timed on the batch.
+std::mutex g_tune_mu;
+std::map<std::vector<uint64_t>, std::pair<size_t, int>> g_tuned;
+
+std::vector<uint64_t> signature(const Batch &bt) {
+  std::vector<uint64_t> sig{uint64_t(bt.dbl)};
+  for (const auto &j : bt.jobs) sig.push_back((uint64_t(j.kind) << 60) ^ j.n);
+  return sig;
+}
+
+void autotune(Batch &bt, hipStream_t s) {
+  const auto sig = signature(bt);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(sig);
+    if (it != g_tuned.end()) {
+      bt.b = build_any(bt, it->second.first);
+      bt.grid = it->second.second;
+      bt.autotuned = 2;
+      return;
+    }
+  }
+  const size_t cap = lds_limit() - 8192;
+  double best = 1e300;
+  size_t best_budget = 0;
+  int best_grid = 0;
+  for (size_t budget : {size_t(32768), size_t(65536), std::min<size_t>(98304, cap)}) {
+    std::unique_ptr<Built> cand;
+    try {
+      cand = build_any(bt, budget);
+    } catch (const Error &) {
+      continue;
+    }
+    std::swap(bt.b, cand);
+    const int per_cu = std::max(1, int((lds_limit() - 8192) / std::max<size_t>(bt.b->lds_need, 1)));
+    for (int wpc : {1, 2, 4}) {
+      if (wpc > per_cu) continue;
+      const int grid = cu_count() * wpc;
+      const double ms = time_ms(bt, grid, s, 3);
+      if (ms < best) {
+        best = ms;
+        best_budget = budget;
+        best_grid = grid;
+      }
+    }
+    std::swap(bt.b, cand);
+  }
+  if (!best_budget) RFH_THROW(3, "ragged batch: no LDS budget fits every job");
+  bt.b = build_any(bt, best_budget);
+  bt.grid = best_grid;
+  bt.autotuned = 1;
+  bt.tuned_ms = best;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned[sig] = {best_budget, best_grid};
+}
+
+template <typename F>
+int guarded_int(F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (const Error &e) {
+    set_error(e.code, e.msg);
+    return e.code;
+  } catch (const std::exception &e) {
+    set_error(1, e.what());
+    return 1;
+  }
+}
+
+}  // namespace
+}  // namespace rfh
+
+using namespace rfh;
+
+extern "C" {
+
+int rfh_batch_create(const rfh_job_t *jobs, int64_t njobs, bool double_precision, bool tune,
+                     void *stream, void **handle) {
+  return guarded_int([&] {
+    auto *bt = new Batch();
+    bt->dbl = double_precision;
+    bt->jobs.assign(jobs, jobs + njobs);
+    const size_t al = double_precision ? 16 : 8;
+    for (const auto &j : bt->jobs) {
+      if (j.kind < RFH_JOB_C2C || j.kind > RFH_JOB_C2R) {
+        delete bt;
+        RFH_THROW(3, "ragged batch: unknown job kind");
+      }
+      if ((reinterpret_cast<uintptr_t>(j.in) | reinterpret_cast<uintptr_t>(j.out)) % al) {
+        delete bt;
+        RFH_THROW(3, "ragged batch: job buffers must be aligned to the complex element size");
+      }
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    try {
+      if (tune) {
+        autotune(*bt, s);
+      } else {
+        bt->b = build_any(*bt, 65536);
+        bt->grid = cu_count() * 2;
+      }
+    } catch (...) {
+      delete bt;
+      throw;
+    }
+    *handle = bt;
+  });
+}
+
+int rfh_batch_execute(void *handle, void *stream) {
+  return guarded_int([&] {
+    Batch *bt = static_cast<Batch *>(handle);
+    run(*bt, bt->grid, static_cast<hipStream_t>(stream));
+  });
+}
+
+int rfh_batch_execute_serial(void *handle, void *stream) {
+  // one drop-in call per transform, as a caller of numba_c2c / numba_r2c /
+  // numba_c2r would make them, enqueued on the same stream
+  hipStream_t prev_s;
+  bool prev_b;
+  thread_stream(&prev_s, &prev_b);
+  rfh_set_stream(stream, false);
+  const int rc = guarded_int([&] {
+    Batch *bt = static_cast<Batch *>(handle);
+    const int64_t rs = bt->dbl ? 8 : 4, cs = 2 * rs;
+    struct Arr1 {
+      arystruct_t s;
+      int64_t ss[2];
+    };
+    uint64_t axis0 = 0;
+    Arr1 ax{};
+    ax.s.nitems = 1;
+    ax.s.itemsize = 8;
+    ax.s.data = &axis0;
+    ax.s.shape_and_strides[0] = 1;
+    ax.s.shape_and_strides[1] = 8;
+    for (const rfh_job_t &j : bt->jobs) {
+      if (j.n == 0) continue;
+      Arr1 ai{}, ao{};
+      const bool c2c = j.kind == RFH_JOB_C2C, r2c = j.kind == RFH_JOB_R2C;
+      const int64_t h = int64_t(j.n / 2 + 1);
+      ai.s.itemsize = (c2c || !r2c) ? cs : rs;
+      ai.s.shape_and_strides[0] = (c2c || r2c) ? int64_t(j.n) : h;
+      ai.s.shape_and_strides[1] = ai.s.itemsize;
+      ai.s.data = const_cast<void *>(j.in);
+      ao.s.itemsize = (c2c || r2c) ? cs : rs;
+      ao.s.shape_and_strides[0] = (c2c || !r2c) ? int64_t(j.n) : h;
+      ao.s.shape_and_strides[1] = ao.s.itemsize;
+      ao.s.data = j.out;
+      if (c2c)
+        numba_c2c(1, &ai.s, &ao.s, &ax.s, j.forward != 0, j.fct, 1);
+      else if (r2c)
+        numba_r2c(1, &ai.s, &ao.s, &ax.s, j.forward != 0, j.fct, 1);
+      else
+        numba_c2r(1, &ai.s, &ao.s, &ax.s, j.forward != 0, j.fct, 1);
+      char msg[256];
+      if (rfh_last_error(msg, sizeof msg)) RFH_THROW(3, std::string("per-transform call: ") + msg);
+    }
+  });
+  rfh_set_stream(prev_s, prev_b);
+  return rc;
+}
+
+int rfh_batch_info(void *handle, int *grid, int *lds_bytes, int *autotuned, double *tuned_ms) {
+  Batch *bt = static_cast<Batch *>(handle);
+  *grid = bt->grid;
+  *lds_bytes = bt->b ? bt->b->lds_need : 0;
+  *autotuned = bt->autotuned;
+  *tuned_ms = bt->tuned_ms;
+  return 0;
+}
+
+int rfh_batch_destroy(void *handle) {
+  delete static_cast<Batch *>(handle);
+  return 0;
+}
+
+}  // extern "C"

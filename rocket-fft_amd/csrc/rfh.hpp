@@ -26,6 +26,8 @@ constexpr int kThreads = 256;
 // ------------------------------------------------------------------ errors
 
 void set_error(int code, const std::string &msg);
+// the calling thread's stream settings (rfh_set_stream)
+void thread_stream(hipStream_t *s, bool *blocking);
 struct Error {
   int code;
   std::string msg;
@@ -106,13 +108,16 @@ struct RealPlan {
 // process-wide switch of the specialised power-of-two kernels (fast.hip)
 extern bool g_fast_enabled;
 
-std::shared_ptr<C2CPlan> get_c2c_plan(uint64_t n, bool dbl);
-std::shared_ptr<RealPlan> get_real_plan(uint64_t n, bool dbl);
+// Plans whose LDS passes fit `lds_budget` bytes of dynamic LDS (0 = the
+// device limit minus the kernels' static LDS).
+std::shared_ptr<C2CPlan> get_c2c_plan(uint64_t n, bool dbl, size_t lds_budget = 0);
+std::shared_ptr<RealPlan> get_real_plan(uint64_t n, bool dbl, size_t lds_budget = 0);
 
 // Radix schedule (execution order) for an LDS-resident FFT of length n.
 std::vector<int> gpu_radices(uint64_t n);
-// Largest line length that fits the LDS budget with `lines` lines per group.
-uint64_t direct_max(bool dbl, int lines);
+// Largest line length that fits the LDS budget (0 = device) with `lines`
+// lines per group.
+uint64_t direct_max(bool dbl, int lines, size_t budget = 0);
 size_t lds_limit();
 
 // Reference plan facts restated (bit-exact).
@@ -184,6 +189,10 @@ struct RealPrePost {
   bool conj = false;  // pre: conjugate input spectrum; post: conjugate output
   bool flip = false;  // negate entries 2, 4, 6, ... (FFTPACK sign rule)
 };
+// pre -> FFT of length rr.m -> post in one LDS pass; needs p.kind == kDirect
+// with p.n == rr.m.
+void launch_r2r_fused(const C2CPlan &p, const LineSet &g, PreMode pre, PostMode post,
+                      const RealPrePost &rr, bool fft_forward, double fct, hipStream_t s);
 void launch_pre(PreMode mode, const RealPrePost &rp, const LineSet &g, bool dbl,
                 void *tmp_complex, hipStream_t s);
 void launch_post(PostMode mode, const RealPrePost &rp, const LineSet &g, bool dbl,

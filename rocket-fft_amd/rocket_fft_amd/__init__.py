@@ -328,3 +328,108 @@ def gpu_schedule(n, double=False):
     if k < 0:
         raise RuntimeError(last_error()[1])
     return kind.value, list(rad[:k]), n1.value, n2.value
+
+
+# ----------------------------------------------------------- ragged batches
+
+class _Job(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("forward", ctypes.c_int32), ("n", ctypes.c_uint64),
+                ("inp", ctypes.c_void_p), ("out", ctypes.c_void_p), ("fct", ctypes.c_double)]
+
+
+JOB_KINDS = {"c2c": 0, "r2c": 1, "c2r": 2}
+
+
+class RaggedBatch:
+    """Many independent transforms of uneven lengths and kinds, planned once
+    and run in one persistent launch (``rfh_batch_*`` in the C ABI).
+
+    jobs: iterable of (kind, forward, n, inp, out, fct) with kind in
+    {"c2c", "r2c", "c2r"}, n the transform length (the real length for
+    r2c/c2r) and inp/out contiguous CUDA tensors holding one line each:
+    c2c n complex -> n complex; r2c n real -> n//2+1 complex; c2r n//2+1
+    complex -> n real.  All jobs share one precision (double=True for
+    complex128/float64).  The tensors must stay alive while the batch is used.
+    """
+
+    def __init__(self, jobs, double=False, autotune=True):
+        import torch
+        L = _load()
+        self._lib = L
+        self._keep = []
+        arr = (_Job * max(1, len(jobs)))()
+        for i, (kind, forward, n, inp, out, fct) in enumerate(jobs):
+            _check_bool("forward", forward)
+            arr[i].kind = JOB_KINDS[kind]
+            arr[i].forward = int(bool(forward))
+            arr[i].n = int(n)
+            arr[i].inp = inp.data_ptr()
+            arr[i].out = out.data_ptr()
+            arr[i].fct = float(fct)
+            self._keep += [inp, out]
+        self.njobs = len(jobs)
+        h = ctypes.c_void_p()
+        stream = torch.cuda.current_stream().cuda_stream
+        L.rfh_batch_create.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_bool,
+                                       ctypes.c_bool, ctypes.c_void_p,
+                                       ctypes.POINTER(ctypes.c_void_p)]
+        rc = L.rfh_batch_create(ctypes.addressof(arr), len(jobs), bool(double), bool(autotune),
+                                ctypes.c_void_p(stream), ctypes.byref(h))
+        if rc:
+            raise RuntimeError("rfh_batch_create: " + last_error()[1])
+        self._h = h
+
+    def _run(self, fname):
+        import torch
+        fn = getattr(self._lib, fname)
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        rc = fn(self._h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        if rc:
+            raise RuntimeError(f"{fname}: " + last_error()[1])
+
+    def execute(self):
+        """One persistent launch over all jobs, on torch's current stream."""
+        self._run("rfh_batch_execute")
+
+    def execute_serial(self):
+        """The same jobs, one drop-in call per transform (comparison point)."""
+        self._run("rfh_batch_execute_serial")
+
+    def info(self):
+        g, l, a, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._lib.rfh_batch_info.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
+        self._lib.rfh_batch_info(self._h, ctypes.byref(g), ctypes.byref(l), ctypes.byref(a),
+                                 ctypes.byref(ms))
+        return {"grid": g.value, "lds_bytes": l.value, "autotuned": a.value,
+                "tuned_ms": ms.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.rfh_batch_destroy.argtypes = [ctypes.c_void_p]
+            self._lib.rfh_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ragged_workload(njobs=4096, seed=5, lo=64, hi=65536, good_frac=0.7, c2c_frac=0.6):
+    """The ragged-batch workload: lengths log-uniform in [lo, hi] from
+    RandomState(seed); good_frac of them rounded up with good_size (complex
+    rule for c2c jobs, real rule for r2c jobs), the rest kept (primes
+    included); c2c_frac complex c2c jobs, the rest real r2c jobs.
+    Returns a list of (kind, n)."""
+    rs = np.random.RandomState(seed)
+    raw = np.exp(rs.uniform(np.log(lo), np.log(hi), njobs)).astype(np.int64)
+    good = rs.uniform(size=njobs) < good_frac
+    kind_c2c = rs.uniform(size=njobs) < c2c_frac
+    out = []
+    for n, g, c in zip(raw, good, kind_c2c):
+        n = int(max(2, n))
+        if g:
+            n = good_size(n, not c)
+        out.append(("c2c" if c else "r2c", n))
+    return out

@@ -208,3 +208,20 @@ def test_two_pass_pow2_fast(dtype):
     x = crandn(rs, (2, 1 << 18), dtype)
     check("c2c", x, x, np.array([1]), True, 1.0)
     check("c2c", x, x, np.array([1]), False, 1.0 / (1 << 18))
+
+
+@pytest.mark.parametrize("kind,type,n", [("dct", 2, 4099), ("dst", 3, 4099), ("dct", 4, 8192),
+                                         ("dct", 1, 6000), ("dst", 1, 5000)])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_dct_long_unfused(kind, type, n, dtype):
+    """Work lengths that need Bluestein or two passes run pre -> c2c -> post."""
+    rs = np.random.RandomState(n)
+    x = rrandn(rs, (3, n), dtype)
+    check(kind, x, x, np.array([1]), type, 1.0, True, tol=2 * tol_of(dtype))
+
+
+def test_separable_hartley_strided_axis0():
+    rs = np.random.RandomState(12)
+    x = rrandn(rs, (300, 7), np.float64)
+    check("r2r_separable_hartley", x, x, np.array([0]), 1.0)
+    check("r2r_fftpack", x, x, np.array([0]), True, True, 1.0)

@@ -1,0 +1,128 @@
+"""Ragged batches: many transforms of uneven lengths and kinds in one
+persistent launch.  Every job is checked against the oracle, the persistent
+launch against the per-transform calls, and graph replays against the eager
+result (several replays with eager work and synchronisation in between)."""
+import numpy as np
+import pytest
+import torch
+
+import rocket_fft_amd as R
+from helpers import oracle_out, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+LENGTHS = [64, 100, 127, 256, 1000, 1536, 2187, 4096, 4099, 4913, 8192, 9973, 12000,
+           20000, 65536, 65537, 3, 2, 1, 17]
+
+
+def make_jobs(dtype_c, lengths, rs):
+    dtype_r = np.float32 if dtype_c == np.complex64 else np.float64
+    jobs, host = [], []
+    for i, n in enumerate(lengths):
+        kind = ("c2c", "r2c", "c2r")[i % 3]
+        fwd = bool(i % 2)
+        fct = 1.0 if i % 4 else 0.5
+        if kind == "c2c":
+            x = (rs.standard_normal(n) + 1j * rs.standard_normal(n)).astype(dtype_c)
+            y = np.zeros(n, dtype_c)
+        elif kind == "r2c":
+            x = rs.standard_normal(n).astype(dtype_r)
+            y = np.zeros(n // 2 + 1, dtype_c)
+        else:
+            h = n // 2 + 1
+            x = (rs.standard_normal(h) + 1j * rs.standard_normal(h)).astype(dtype_c)
+            y = np.zeros(n, dtype_r)
+        tin = torch.from_numpy(x).cuda()
+        tout = torch.from_numpy(y).cuda()
+        jobs.append((kind, fwd, n, tin, tout, fct))
+        host.append((kind, fwd, n, x, y, fct))
+    return jobs, host
+
+
+def expected(host):
+    out = []
+    for kind, fwd, n, x, y, fct in host:
+        out.append(oracle_out(kind, x, y, np.array([0]), fwd, fct))
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("autotune", [False, True])
+def test_ragged_matches_oracle(dtype, autotune):
+    rs = np.random.RandomState(5)
+    jobs, host = make_jobs(dtype, LENGTHS, rs)
+    want = expected(host)
+    tol = 3e-6 if dtype == np.complex64 else 1e-13
+    print(f"ragged create dtype={dtype.__name__} autotune={autotune}", flush=True)
+    b = R.RaggedBatch(jobs, double=(dtype == np.complex128), autotune=autotune)
+    print("ragged info", b.info(), flush=True)
+    for j in jobs:
+        j[4].zero_()
+    b.execute()
+    torch.cuda.synchronize()
+    for (kind, fwd, n, tin, tout, fct), w in zip(jobs, want):
+        e = rel_l2(tout.cpu().numpy(), w)
+        assert e <= tol, f"ragged {kind} n={n} fwd={fwd}: {e:.3e}"
+    got = [j[4].cpu().numpy().copy() for j in jobs]
+    for j in jobs:
+        j[4].zero_()
+    b.execute_serial()
+    torch.cuda.synchronize()
+    for (kind, fwd, n, tin, tout, fct), g in zip(jobs, got):
+        e = rel_l2(tout.cpu().numpy(), g)
+        assert e <= tol, f"serial vs persistent {kind} n={n}: {e:.3e}"
+    b.close()
+
+
+def test_ragged_graph_replays():
+    rs = np.random.RandomState(6)
+    jobs, host = make_jobs(np.complex64, LENGTHS[:12], rs)
+    b = R.RaggedBatch(jobs, double=False, autotune=False)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        b.execute()
+    torch.cuda.synchronize()
+    eager = [j[4].cpu().numpy().copy() for j in jobs]
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        b.execute()
+    for rep in range(3):
+        for j in jobs:
+            j[4].zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for (kind, fwd, n, tin, tout, fct), e in zip(jobs, eager):
+            assert rel_l2(tout.cpu().numpy(), e) < 1e-6, f"replay {rep} {kind} n={n}"
+    b.close()
+
+
+def test_ragged_workload_sample():
+    """A 256-job slice of the ragged benchmark workload (log-uniform lengths,
+    good sizes and raw lengths, c2c and r2c) through the persistent launch,
+    checked against the per-transform calls."""
+    spec = R.ragged_workload(njobs=256, seed=5)
+    jobs = []
+    for kind, n in spec:
+        if kind == "c2c":
+            x = torch.randn(n, dtype=torch.complex64, device="cuda")
+            y = torch.empty(n, dtype=torch.complex64, device="cuda")
+        else:
+            x = torch.randn(n, dtype=torch.float32, device="cuda")
+            y = torch.empty(n // 2 + 1, dtype=torch.complex64, device="cuda")
+        jobs.append((kind, True, n, x, y, 1.0))
+    b = R.RaggedBatch(jobs, double=False, autotune=True)
+    b.execute()
+    torch.cuda.synchronize()
+    got = [j[4].clone() for j in jobs]
+    b.execute_serial()
+    torch.cuda.synchronize()
+    worst = max(float(torch.linalg.norm(g - j[4]) / torch.linalg.norm(j[4]))
+                for g, j in zip(got, jobs))
+    assert worst < 3e-6, worst
+    # spot-check against torch.fft in FP64 (independent of this engine)
+    for kind, fwd, n, x, y, fct in jobs[:16]:
+        ref = (torch.fft.fft(x.to(torch.complex128)) if kind == "c2c"
+               else torch.fft.rfft(x.to(torch.float64)))
+        assert float(torch.linalg.norm(y.to(ref.dtype) - ref) / torch.linalg.norm(ref)) < 3e-6
+    b.close()
