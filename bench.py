@@ -139,6 +139,52 @@ def probe(torch, R):
     print("probe done", flush=True)
 
 
+def precision_config(torch, R, stream):
+    """complex64 c2c of 16384 lines of 4096: plain FP32 butterflies against the
+    matrix-core forms, GB/s and rel-L2 against an FP64 transform of the same
+    input (torch.fft in complex128), on both input sets: iid N(0,1) and
+    magnitudes log-uniform over 1e-3..1e3 with uniform phase."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4)
+    shape = (16384, 4096)
+    sets = {"normal": torch.randn(shape, dtype=torch.complex64, device="cuda", generator=g)}
+    mag = torch.exp(torch.empty(shape, device="cuda").uniform_(np.log(1e-3), np.log(1e3),
+                                                               generator=g))
+    ph = torch.empty(shape, device="cuda").uniform_(0, 2 * np.pi, generator=g)
+    sets["loguniform"] = torch.polar(mag, ph).to(torch.complex64)
+    del mag, ph
+    out = {}
+    nbytes = 2 * shape[0] * shape[1] * 8
+    ax = np.array([1])
+    for mode in ("fp32", "mfma_fp16", "mfma_bf16", "mfma_bf16_2split"):
+        prev = R.set_precision(mode)
+        try:
+            entry = {}
+            for name, x in sets.items():
+                y = torch.empty_like(x)
+                with torch.cuda.stream(stream):
+                    R.c2c(x, y, ax, True, 1.0, 1)
+                torch.cuda.synchronize()
+                ms, _ = graph_time(torch, lambda: R.c2c(x, y, ax, True, 1.0, 1), stream, 10)
+                err = 0.0
+                for lo in range(0, shape[0], 2048):  # FP64 reference in slices
+                    ref = torch.fft.fft(x[lo:lo + 2048].to(torch.complex128), dim=1)
+                    err += float(torch.sum(torch.abs(y[lo:lo + 2048].to(torch.complex128) - ref) ** 2))
+                    den = float(torch.sum(torch.abs(ref) ** 2))
+                    entry.setdefault("_den_" + name, 0.0)
+                    entry["_den_" + name] += den
+                entry[name] = {"ms": round(ms, 4), "GB/s": round(nbytes / ms / 1e6, 1),
+                               "rel_l2_vs_fp64": float(np.sqrt(err / entry.pop("_den_" + name)))}
+                del y
+            out[mode] = entry
+        finally:
+            R.set_precision(prev)
+    base = out["fp32"]["normal"]["ms"]
+    for mode in out:
+        out[mode]["speedup_vs_fp32"] = round(base / out[mode]["normal"]["ms"], 3)
+    return out
+
+
 def ragged_config(torch, R, stream):
     """4096 transforms of log-uniform lengths 64..65536 (70% good sizes, 30%
     raw incl. primes; 60% c2c complex64, 40% r2c float32) in one persistent
@@ -305,6 +351,7 @@ def main():
     if not args.no_extra:
         res["configs"] = extra_configs(torch, R, stream, 10)
         res["configs"]["ragged_4096_c64_r2c"] = ragged_config(torch, R, stream)
+        res["configs"]["precision_c64_16384x4096"] = precision_config(torch, R, stream)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(x_h, d_h, args.cpu_seconds)
     print(json.dumps(res))

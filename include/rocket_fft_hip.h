@@ -111,6 +111,19 @@ int rfh_set_stream(void *stream, bool blocking);
  * kernels.  Process-wide; meant for tests that compare the two paths. */
 int rfh_set_fast_path(bool enabled);
 
+/* ---- split-precision form ---------------------------------------------------
+ * Process-wide arithmetic of complex64 transforms of length 4096 on
+ * contiguous lines: 0 = plain FP32 butterflies (default), 1 = radix-16 stages
+ * as DFT matrix products on FP16 matrix cores with hi/lo operand splits and
+ * FP32 accumulation, 2 = the same on BF16 with hi/mid/lo splits, 3 = BF16 with
+ * hi/lo splits only (precision study; about 1e-5).  No reference
+ * counterpart.  Returns the previous mode. */
+int rfh_set_precision(int mode);
+/* One 16x16x32 matrix-core product on device buffers laid out as the
+ * hardware operands (a, b: 64 lanes x 8 floats, d: 64 x 4; bf16 != 0 selects
+ * BF16, else FP16), for tests of the operand layout. */
+int rfh_mfma_probe(int bf16, const float *a, const float *b, float *d, void *stream);
+
 /* ---- ragged batches ---------------------------------------------------------
  * Many independent transforms of uneven lengths and kinds (complex c2c, real
  * r2c / c2r), planned once and executed in one persistent launch (a job

@@ -719,4 +719,11 @@ bool launch_fast(int op, const dev::LineArgs<T> &a, bool lf_in, bool lf_out, boo
                  bool vout, hipStream_t s);
 extern bool g_fast_enabled;
 
+// Split-precision MFMA kernels (mfma.hip); false when the mode is off or the
+// pass is not a contiguous complex64 line of 4096.
+bool launch_mfma(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out, bool vin,
+                 bool vout, hipStream_t s);
+void mfma_probe(int bf, const float *a, const float *b, float *d, hipStream_t s);
+extern int g_precision_mode;
+
 }  // namespace rfh

@@ -23,7 +23,7 @@
 #include <mutex>
 
 #include "../../include/rocket_fft_hip.h"
-#include "rfh.hpp"
+#include "device.hpp"
 
 namespace rfh {
 
@@ -695,6 +695,19 @@ int rfh_set_stream(void *stream, bool blocking) {
   t_stream = static_cast<hipStream_t>(stream);
   t_blocking = blocking;
   return 0;
+}
+
+int rfh_set_precision(int mode) {
+  const int prev = g_precision_mode;
+  if (mode >= 0 && mode <= 3) g_precision_mode = mode;
+  return prev;
+}
+
+int rfh_mfma_probe(int bf16, const float *a, const float *b, float *d, void *stream) {
+  int rc = 0;
+  guarded([&] { mfma_probe(bf16, a, b, d, static_cast<hipStream_t>(stream)); });
+  rc = t_err;
+  return rc;
 }
 
 int rfh_set_fast_path(bool enabled) {

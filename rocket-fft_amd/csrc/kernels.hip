@@ -181,6 +181,8 @@ void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
     vout = g.eso == 1 && pairs_aligned(g.out, sizeof(T), 0, g.bso, g.nb);
   else
     vout = pairs_aligned(g.out, sizeof(T), g.eso, g.bso, g.nb);
+  if constexpr (sizeof(T) == 4)
+    if (launch_mfma(ps.op, a, a.lfast_in, a.lfast_out, vin, vout, s)) return;
   if (launch_fast<T>(ps.op, a, a.lfast_in, a.lfast_out, vin, vout, s)) return;
   switch (ps.op) {
     case OP_C2C: launch_var<T, OP_C2C>(a, lds, vin, vout, s); break;

@@ -1,0 +1,259 @@
+// mfma.hip -- the split-precision form: radix-16 DFT stages as dense 16x16
+// complex matrix products on the gfx950 matrix cores.
+//
+// A radix-16 butterfly is X = F v with F[k][r] = e^{dir 2 pi i k r / 16}.
+// Sixteen butterflies of a wave are one matrix product D (16x16) = A (16x32)
+// x B (32x16) with the complex product embedded in real arithmetic:
+//   B = [Re v; Im v] (one column per butterfly),
+//   A_re = [Re F, -Im F]  ->  D_re = Re X,     A_im = [Im F, Re F]  ->  D_im = Im X.
+// FP16 form: every operand is split x = hi + lo (two FP16 values, ~22 bits),
+// and D accumulates hi*hi + hi*lo + lo*hi in FP32 (v_mfma_f32_16x16x32_f16);
+// each butterfly is first scaled by a power of two so its largest input is
+// just below 2^10 (exact, undone after the product), keeping FP16 in range.
+// BF16 form: bf16 has FP32's range but 8 mantissa bits, so operands are
+// split three ways (hi + mid + lo) and the six products down to 2^-24 are
+// accumulated (v_mfma_f32_16x16x32_bf16); a two-way BF16 split (three
+// products, ~16 bits) is kept as a precision-study mode.
+//
+// Operand layout of v_mfma_f32_16x16x32_{f16,bf16} (64 lanes; lane l, g =
+// l / 16): A row l % 16, columns 8g..8g+7; B rows 8g..8g+7, column l % 16;
+// D rows 4g..4g+3, column l % 16.  k_mfma_probe lets a GPU test check this
+// layout against a host matrix product.
+//
+// The kernel covers complex64 lines of 4096 (three radix-16 stages, 256
+// threads per line, one line per workgroup); everything else about it
+// (global loads of stage 0, LDS exchanges with the padded layout, twiddle
+// tables, stores of the last stage) is the specialised kernel of fast.hip.
+#include "passes.hpp"
+
+namespace rfh {
+
+int g_precision_mode = 0;  // 0 plain FP32, 1 FP16 split, 2 BF16 3-split, 3 BF16 2-split
+
+namespace {
+using namespace dev;
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf168 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+enum { MODE_F16 = 1, MODE_BF16_3 = 2, MODE_BF16_2 = 3 };
+
+__global__ void k_mfma_probe(int bf, const float *a, const float *b, float *d) {
+  // a, b: 64 lanes x 8 values in the operand layout above; d: 64 x 4
+  const int l = threadIdx.x;
+  f4 c = {0.f, 0.f, 0.f, 0.f};
+  f4 r;
+  if (bf) {
+    bf168 x, y;
+    for (int i = 0; i < 8; ++i) {
+      x[i] = (__bf16)a[l * 8 + i];
+      y[i] = (__bf16)b[l * 8 + i];
+    }
+    r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, y, c, 0, 0, 0);
+  } else {
+    half8 x, y;
+    for (int i = 0; i < 8; ++i) {
+      x[i] = (_Float16)a[l * 8 + i];
+      y[i] = (_Float16)b[l * 8 + i];
+    }
+    r = __builtin_amdgcn_mfma_f32_16x16x32_f16(x, y, c, 0, 0, 0);
+  }
+  for (int i = 0; i < 4; ++i) d[l * 4 + i] = r[i];
+}
+
+template <int MODE>
+struct Ops;
+
+template <>
+struct Ops<MODE_F16> {
+  typedef half8 V;
+  static constexpr int terms = 2;
+  __device__ static V split(const float *x, int t) {
+    V v;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const _Float16 h = (_Float16)x[i];
+      v[i] = t == 0 ? h : (_Float16)(x[i] - (float)h);
+    }
+    return v;
+  }
+  __device__ static f4 mma(V a, V b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+
+template <int TERMS>
+struct OpsBf {
+  typedef bf168 V;
+  static constexpr int terms = TERMS;
+  __device__ static V split(const float *x, int t) {
+    V v;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      float r = x[i];
+      __bf16 p = (__bf16)r;
+      for (int s = 0; s < t; ++s) {
+        r -= (float)p;
+        p = (__bf16)r;
+      }
+      v[i] = p;
+    }
+    return v;
+  }
+  __device__ static f4 mma(V a, V b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Ops<MODE_BF16_3> : OpsBf<3> {};
+template <>
+struct Ops<MODE_BF16_2> : OpsBf<2> {};
+
+__device__ __forceinline__ int padf(int i) { return i + (i >> 4); }
+
+// Largest magnitude over the four lanes (l, l^16, l^32, l^48) that hold
+// one butterfly.
+__device__ __forceinline__ float group_max(float m) {
+  m = fmaxf(m, __shfl_xor(m, 16));
+  m = fmaxf(m, __shfl_xor(m, 32));
+  return m;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_fft4096_mfma(const LineArgs<float> a) {
+  using O = Ops<MODE>;
+  typedef typename O::V V;
+  constexpr int N = 4096, NR = 256;
+  __shared__ C2<float> line[N + (N >> 4) + 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = lane & 15, kg = lane >> 4;
+  const int part = kg >> 1, rbase = (kg & 1) * 8;
+  const float dir = a.dir;
+  const int64_t b = xcd_tile();
+  const bool valid = b < a.nlines;
+  const float *gin = a.in + b * a.bsi[a.nb > 0 ? a.nb - 1 : 0];
+  float *gout = a.out + b * a.bso[a.nb > 0 ? a.nb - 1 : 0];
+
+  // A operands: rows = output k (lane % 16), columns 8*kg..8*kg+7 of [Re|Im]
+  V Are[O::terms], Aim[O::terms];
+  {
+    float re[8], im[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int kk = 8 * kg + i, r = kk & 15;
+      float s, c;
+      sincospif(float((col * r) & 15) / 8.0f, &s, &c);  // angle 2 pi k r / 16
+      const float fr = c, fi = dir * s;
+      re[i] = kk < 16 ? fr : -fi;
+      im[i] = kk < 16 ? fi : fr;
+    }
+#pragma unroll
+    for (int t = 0; t < O::terms; ++t) {
+      Are[t] = O::split(re, t);
+      Aim[t] = O::split(im, t);
+    }
+  }
+
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const int NS = s == 0 ? 1 : (s == 1 ? 16 : 256);
+    const C2<float> *tw = a.tw + a.f.st[s].twoff;
+    f4 dre[4], dim[4];
+    float inv[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int j = wave * 64 + g * 16 + col;
+      const int k = j % NS;
+      float x[8];
+      float m = 0.f;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int r = rbase + i;
+        C2<float> v;
+        if (s == 0)
+          v = valid ? ldc<true>(gin + 2 * (j + r * NR)) : C2<float>{0.f, 0.f};
+        else
+          v = line[padf(j + r * NR)];
+        if (s > 0 && r > 0) v = cmul(v, dirw(tw[(r - 1) * NS + k], dir));
+        x[i] = part ? v.y : v.x;
+        m = fmaxf(m, fabsf(x[i]));
+      }
+      m = group_max(m);
+      int ex = 0;
+      (void)frexpf(m, &ex);
+      const float sc = m > 0.f ? ldexpf(1.f, 10 - ex) : 1.f;
+      inv[g] = 1.f / sc;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) x[i] *= sc;
+      V B[O::terms];
+#pragma unroll
+      for (int t = 0; t < O::terms; ++t) B[t] = O::split(x, t);
+      f4 zr = {0.f, 0.f, 0.f, 0.f}, zi = {0.f, 0.f, 0.f, 0.f};
+      // products down to the split's precision: sum of term indices < terms
+#pragma unroll
+      for (int ta = 0; ta < O::terms; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < O::terms; ++tb)
+          if (ta + tb < O::terms) {
+            zr = O::mma(Are[ta], B[tb], zr);
+            zi = O::mma(Aim[ta], B[tb], zi);
+          }
+      dre[g] = zr;
+      dim[g] = zi;
+    }
+    if (s == 2) {
+      if (valid) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int j = wave * 64 + g * 16 + col;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int r = 4 * kg + q;
+            stc<true>(gout + 2 * (j + r * NR),
+                      C2<float>{a.fct * dre[g][q] * inv[g], a.fct * dim[g][q] * inv[g]});
+          }
+        }
+      }
+    } else {
+      __syncthreads();  // every read of this stage is done
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int j = wave * 64 + g * 16 + col;
+        const int k = j % NS;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = 4 * kg + q;
+          line[padf((j - k) * 16 + k + r * NS)] = C2<float>{dre[g][q] * inv[g], dim[g][q] * inv[g]};
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace
+
+// Complex64 lines of 4096, contiguous, one line per group: the MFMA form.
+bool launch_mfma(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out, bool vin,
+                 bool vout, hipStream_t s) {
+  if (!g_precision_mode || op != OP_C2C || a.f.n != 4096 || lf_in || lf_out || !vin || !vout)
+    return false;
+  if (a.f.nst != 3 || a.esi != 2 || a.eso != 2 || a.nb > 1) return false;
+  const unsigned grid = unsigned(a.nlines);
+  switch (g_precision_mode) {
+    case MODE_F16: hipLaunchKernelGGL(k_fft4096_mfma<MODE_F16>, dim3(grid), dim3(256), 0, s, a); break;
+    case MODE_BF16_3: hipLaunchKernelGGL(k_fft4096_mfma<MODE_BF16_3>, dim3(grid), dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_fft4096_mfma<MODE_BF16_2>, dim3(grid), dim3(256), 0, s, a); break;
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) RFH_THROW(2, std::string("mfma launch: ") + hipGetErrorString(e));
+  return true;
+}
+
+void mfma_probe(int bf, const float *a, const float *b, float *d, hipStream_t s) {
+  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, s, bf, a, b, d);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace rfh

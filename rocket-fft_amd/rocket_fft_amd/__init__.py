@@ -311,6 +311,36 @@ def plan_facts(n):
             "blue_r": (bool(br), int(n2r.value))}
 
 
+PRECISION_MODES = {"fp32": 0, "mfma_fp16": 1, "mfma_bf16": 2, "mfma_bf16_2split": 3}
+
+
+def set_precision(mode):
+    """Arithmetic of complex64 length-4096 transforms: "fp32" (plain
+    butterflies), "mfma_fp16" (FP16 matrix cores, hi/lo splits), "mfma_bf16"
+    (BF16, hi/mid/lo), "mfma_bf16_2split" (BF16 hi/lo, study).  Returns the
+    previous mode's name."""
+    L = _load()
+    L.rfh_set_precision.argtypes = [ctypes.c_int]
+    L.rfh_set_precision.restype = ctypes.c_int
+    prev = L.rfh_set_precision(PRECISION_MODES[mode])
+    return {v: k for k, v in PRECISION_MODES.items()}[prev]
+
+
+def mfma_probe(a_lanes, b_lanes, bf16=False):
+    """d (64 x 4) = one 16x16x32 matrix-core product of device operands laid
+    out per lane (64 x 8 each), float32 torch CUDA tensors."""
+    import torch
+    L = _load()
+    d = torch.zeros((64, 4), dtype=torch.float32, device=a_lanes.device)
+    L.rfh_mfma_probe.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_void_p, ctypes.c_void_p]
+    rc = L.rfh_mfma_probe(int(bf16), a_lanes.data_ptr(), b_lanes.data_ptr(), d.data_ptr(),
+                          ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    if rc:
+        raise RuntimeError(last_error()[1])
+    return d
+
+
 def set_fast_path(enabled):
     """Process-wide switch of the specialised power-of-two kernels (tests)."""
     _load().rfh_set_fast_path(bool(enabled))
