@@ -183,7 +183,7 @@ void c2c_axis(Ctx &c, const int64_t *shape, int nd, int ax, const int64_t *si,
   const uint64_t n = uint64_t(shape[ax]);
   auto plan = get_c2c_plan(n, c.dbl);
   LineSet g = lines_for_axis(shape, nd, ax, si, so, in, out);
-  void *tmp = c.ws->get(kSlotTwoPass, c2c_tmp_bytes(*plan, g.nlines));
+  void *tmp = c.ws->get(kSlotTwoPass, c2c_tmp_bytes(*plan, g));
   launch_c2c(*plan, g, forward ? -1 : +1, fct, c.s, tmp);
 }
 
@@ -203,7 +203,7 @@ void *complex_work(Ctx &c, int64_t nlines, int64_t m) {
 void c2c_work(Ctx &c, void *work, int64_t nlines, int64_t m, bool forward) {
   auto plan = get_c2c_plan(uint64_t(m), c.dbl);
   LineSet g = contiguous_lines(work, nlines, m);
-  void *tmp = c.ws->get(kSlotTwoPass, c2c_tmp_bytes(*plan, nlines));
+  void *tmp = c.ws->get(kSlotTwoPass, c2c_tmp_bytes(*plan, g));
   launch_c2c(*plan, g, forward ? -1 : +1, 1.0, c.s, tmp);
 }
 

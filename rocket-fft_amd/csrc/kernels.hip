@@ -264,15 +264,52 @@ void contiguous_like(const LineSet &g, uint64_t len, int64_t *bs) {
 
 }  // namespace
 
-size_t c2c_tmp_bytes(const C2CPlan &p, int64_t nlines) {
+namespace {
+
+// Two-pass transforms run in chunks of lines whose intermediate fits in
+// kChunkBytes, so that pass B reads what pass A just wrote from the Infinity
+// Cache (256 MB on MI355X) instead of HBM; chunking needs a single batch
+// dimension to slice.
+constexpr size_t kChunkBytes = size_t(64) << 20;
+
+int64_t chunk_lines(const C2CPlan &p, const LineSet &g) {
+  if (p.kind != kTwoPass && p.kind != kBlueTwoPass) return g.nlines;
+  const size_t per = size_t(p.n1) * p.n2 * (p.dbl ? 16 : 8) * (p.kind == kBlueTwoPass ? 2 : 1);
+  const int64_t c = std::max<int64_t>(1, int64_t(kChunkBytes / per));
+  return (g.nb == 1 && g.nlines > c) ? c : g.nlines;
+}
+
+LineSet slice_lines(const LineSet &g, int64_t c0, int64_t cn, size_t scalar) {
+  LineSet s = g;
+  s.nlines = cn;
+  s.bdim[0] = cn;
+  s.in = static_cast<const char *>(g.in) + c0 * g.bsi[0] * int64_t(scalar);
+  s.out = static_cast<char *>(g.out) + c0 * g.bso[0] * int64_t(scalar);
+  return s;
+}
+
+}  // namespace
+
+size_t c2c_tmp_bytes(const C2CPlan &p, const LineSet &g) {
   const size_t elem = p.dbl ? 16 : 8;
-  if (p.kind == kTwoPass) return size_t(nlines) * p.n1 * p.n2 * elem;
-  if (p.kind == kBlueTwoPass) return 2 * size_t(nlines) * p.n1 * p.n2 * elem;
+  const size_t lines = size_t(chunk_lines(p, g));
+  if (p.kind == kTwoPass) return lines * p.n1 * p.n2 * elem;
+  if (p.kind == kBlueTwoPass) return 2 * lines * p.n1 * p.n2 * elem;
   return 0;
 }
 
 std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct,
                                      void *tmp) {
+  const int64_t cl = chunk_lines(p, g);
+  if (cl < g.nlines) {  // one two-pass sequence per chunk, sharing tmp
+    std::vector<PassLaunch> out;
+    const size_t scalar = p.dbl ? 8 : 4;
+    for (int64_t c0 = 0; c0 < g.nlines; c0 += cl) {
+      const LineSet s = slice_lines(g, c0, std::min(cl, g.nlines - c0), scalar);
+      for (const PassLaunch &pl : describe_c2c(p, s, dir, fct, tmp)) out.push_back(pl);
+    }
+    return out;
+  }
   std::vector<PassLaunch> out;
   const bool dbl = p.dbl;
   const int sub_conj = dir > 0 ? SUB_CONJ : 0;

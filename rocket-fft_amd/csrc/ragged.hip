@@ -169,7 +169,7 @@ std::unique_ptr<Built> build(const std::vector<rfh_job_t> &jobs, bool dbl, size_
     if (j.kind == RFH_JOB_C2C) {
       cplan[i] = get_c2c_plan(j.n, dbl, budget);
       tmp_off[i] = arena;
-      arena = align_up(arena + c2c_tmp_bytes(*cplan[i], 1), 256);
+      arena = align_up(arena + c2c_tmp_bytes(*cplan[i], one_line(nullptr, nullptr, 2, 2)), 256);
     } else {
       rplan[i] = get_real_plan(j.n, dbl, budget);
       if (!rplan[i]->fused) {  // pre -> c2c(n) -> post through a work line
@@ -177,7 +177,7 @@ std::unique_ptr<Built> build(const std::vector<rfh_job_t> &jobs, bool dbl, size_
         work_off[i] = arena;
         arena = align_up(arena + size_t(j.n) * ce, 256);
         tmp_off[i] = arena;
-        arena = align_up(arena + c2c_tmp_bytes(*cplan[i], 1), 256);
+        arena = align_up(arena + c2c_tmp_bytes(*cplan[i], one_line(nullptr, nullptr, 2, 2)), 256);
       }
     }
   }

@@ -146,7 +146,7 @@ struct LineSet {
 // dir: -1 forward, +1 backward.
 void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStream_t s,
                 void *tmp);
-size_t c2c_tmp_bytes(const C2CPlan &p, int64_t nlines);
+size_t c2c_tmp_bytes(const C2CPlan &p, const LineSet &g);
 
 enum RealOp { kR2C = 0, kC2R = 1 };
 // Fused real transforms: r2c (real lines -> n/2+1 complex, conj when

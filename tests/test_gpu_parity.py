@@ -225,3 +225,22 @@ def test_separable_hartley_strided_axis0():
     x = rrandn(rs, (300, 7), np.float64)
     check("r2r_separable_hartley", x, x, np.array([0]), 1.0)
     check("r2r_fftpack", x, x, np.array([0]), True, True, 1.0)
+
+
+@pytest.mark.parametrize("rows,n,dtype", [(40, 1 << 18, np.complex64), (24, 1 << 18, np.complex128),
+                                          (80, 65537, np.complex64)])
+def test_two_pass_chunked(rows, n, dtype):
+    """Batches whose two-pass intermediate exceeds one 64 MiB chunk run chunk
+    by chunk; checked against an FP64 transform (torch.fft, complex128)."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(rows)
+    tdt = torch.complex64 if dtype == np.complex64 else torch.complex128
+    x = torch.randn((rows, n), dtype=tdt, device="cuda", generator=g)
+    y = torch.empty_like(x)
+    print(f"case chunked c2c rows={rows} n={n} {dtype.__name__}", flush=True)
+    R.c2c(x, y, np.array([1]), True, 1.0, 1)
+    ref = torch.fft.fft(x.to(torch.complex128), dim=1)
+    torch.cuda.synchronize()
+    e = float(torch.linalg.norm(y.to(torch.complex128) - ref) / torch.linalg.norm(ref))
+    assert e <= 3 * tol_of(dtype), e
