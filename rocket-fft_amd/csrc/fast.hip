@@ -88,6 +88,14 @@ __device__ __forceinline__ C2<T> fload(const FCtx<T> &c, int p) {
     const C2<T> A = xj + conjc(xm);
     const C2<T> B = cmul(xj - conjc(xm), conjc(a.t0[p]));
     return C2<T>{A.x - B.y, A.y + B.x};
+  } else if constexpr (OP == OP_PASSA) {
+    if (a.sub & SUB_CHIRP) {  // Bluestein columns: x[m] times the chirp, zero from m = n on
+      const int64_t m = int64_t(p) * a.other + c.aux;
+      if (m >= a.bign) return C2<T>{T(0), T(0)};
+      const C2<T> ch = a.t2[m];
+      return cmul(ldc<VIN>(c.gin + p * a.esi), (a.sub & SUB_CONJ) ? ch : conjc(ch));
+    }
+    return ldc<VIN>(c.gin + p * a.esi);
   } else {
     return ldc<VIN>(c.gin + p * a.esi);
   }

@@ -53,6 +53,29 @@ def test_c2c_bluestein_two_pass(dtype):
     check("c2c", x, x, np.array([1]), False, 1.0 / 65537, tol=3 * tol_of(dtype))
 
 
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_bluestein_two_pass_pow2_columns(dtype):
+    """n = 50174 = 2 * 25087: Bluestein length 100352 = 256 * 392, so the
+    column pass is a power-of-two length and runs on the specialised line
+    kernel, which must apply the chirp (and the zero padding from m = n on)
+    while it loads.  c2c both ways, and r2c / c2r, whose unfused form runs
+    the same c2c of length n."""
+    n = 50174
+    dbl = dtype == np.complex128
+    kind, _, n1, n2 = R.gpu_schedule(n, dbl)
+    print(f"case bluestein pow2 columns n={n} {dtype.__name__} schedule={kind, n1, n2}", flush=True)
+    assert kind == 3 and n1 * n2 == 100352 and n1 & (n1 - 1) == 0 and n1 <= 4096
+    rs = np.random.RandomState(9)
+    x = crandn(rs, (2, n), dtype)
+    check("c2c", x, x, np.array([1]), True, 1.0, tol=3 * tol_of(dtype))
+    check("c2c", x, x, np.array([1]), False, 1.0 / n, tol=3 * tol_of(dtype))
+    xr = rrandn(rs, (2, n), np.float64 if dbl else np.float32)
+    yr = np.zeros((2, n // 2 + 1), dtype)
+    check("r2c", xr, yr, np.array([1]), True, 1.0, tol=3 * tol_of(dtype))
+    xc = crandn(rs, (2, n // 2 + 1), dtype)
+    check("c2r", xc, xr, np.array([1]), False, 1.0 / n, tol=3 * tol_of(dtype))
+
+
 @pytest.mark.parametrize("shape,axes", [((16, 24, 10), (0, 1, 2)), ((16, 24, 10), (2, 0)),
                                         ((33, 17), (0,)), ((33, 17), (1, 0)),
                                         ((5, 6, 7, 8), (3, 1)), ((64, 3), (0,))])
