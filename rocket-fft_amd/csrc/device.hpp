@@ -233,45 +233,66 @@ __device__ __forceinline__ void stage_fixed(const C2<T> *__restrict__ src,
   }
 }
 
-// Any odd radix at run time: inputs are twiddled in place in src (each
-// butterfly owns its inputs), then every output pair is accumulated straight
-// from LDS.  O(R) work per element; used for primes above 13.
+// Any odd radix at run time: inputs are twiddled in place in src, then each
+// output pair (k, R - k) of each butterfly is accumulated straight from LDS by
+// its own thread, from the symmetric sums x_m + x_{R-m} and x_m - x_{R-m}.
+// O(R) work per element, spread over (R + 1) / 2 threads per butterfly so
+// that a line with few butterflies still fills the workgroup; used for
+// primes above 13.
 template <typename T>
 __device__ void stage_generic(C2<T> *__restrict__ src, C2<T> *__restrict__ dst, int L,
                               int ld, int N, int R, int ns, const C2<T> *__restrict__ tw,
                               const C2<T> *__restrict__ g, T dir) {
   const int NR = N / R;
-  const int total = L * NR;
   const int H = (R - 1) / 2;
   const float invNR = 1.0f / float(NR), invns = 1.0f / float(ns);
+  if (ns > 1) {  // stage twiddles, in place, one (butterfly, input) pair per item
+    const int tot = L * NR * (R - 1);
+    const float inv = 1.0f / float(L * NR);
+    for (int u = threadIdx.x; u < tot; u += blockDim.x) {
+      const int r1 = fdiv(u, L * NR, inv);  // r - 1
+      const int lj = u - r1 * (L * NR);
+      const int l = fdiv(lj, NR, invNR);
+      const int j = lj - l * NR;
+      const int k = j - fdiv(j, ns, invns) * ns;
+      C2<T> *p = src + l * ld + j + (r1 + 1) * NR;
+      *p = cmul(*p, dirw(tw[r1 * ns + k], dir));
+    }
+    __syncthreads();
+  }
+  // one (butterfly, output pair kk / R - kk) per item, butterflies fastest so
+  // that a wave shares kk and reads the same root g[kk m mod R]
+  const int total = L * NR * (H + 1);
+  const float invH = 1.0f / float(H + 1);
   for (int u = threadIdx.x; u < total; u += blockDim.x) {
-    const int l = fdiv(u, NR, invNR);
-    const int j = u - l * NR;
+    const int lk = fdiv(u, NR, invNR);
+    const int j = u - lk * NR;
+    const int l = fdiv(lk, H + 1, invH);
+    const int kk = lk - l * (H + 1);
     const int q = fdiv(j, ns, invns);
     const int k = j - q * ns;
-    C2<T> *s = src + l * ld + j;
+    const C2<T> *s = src + l * ld + j;
     C2<T> *d = dst + l * ld + q * ns * R + k;
-    if (ns > 1)
-      for (int r = 1; r < R; ++r) s[r * NR] = cmul(s[r * NR], dirw(tw[(r - 1) * ns + k], dir));
     const C2<T> x0 = s[0];
-    C2<T> sum = x0;
-    for (int r = 1; r < R; ++r) sum = sum + s[r * NR];
-    d[0] = sum;
-    for (int kk = 1; kk <= H; ++kk) {
-      C2<T> a = x0, b = {T(0), T(0)};
-      int idx = 0;
-      for (int m = 1; m <= H; ++m) {
-        idx += kk;
-        if (idx >= R) idx -= R;
-        const C2<T> vm = s[m * NR], vp = s[(R - m) * NR];
-        const C2<T> w = g[idx];
-        a = a + w.x * (vm + vp);
-        b = b + w.y * (vm - vp);
-      }
-      const C2<T> rb = rot(b, dir);
-      d[kk * ns] = a + rb;
-      d[(R - kk) * ns] = a - rb;
+    if (kk == 0) {
+      C2<T> sum = x0;
+      for (int r = 1; r < R; ++r) sum = sum + s[r * NR];
+      d[0] = sum;
+      continue;
     }
+    C2<T> a = x0, b = {T(0), T(0)};
+    int idx = 0;
+    for (int m = 1; m <= H; ++m) {
+      idx += kk;
+      if (idx >= R) idx -= R;
+      const C2<T> vm = s[m * NR], vp = s[(R - m) * NR];
+      const C2<T> w = g[idx];
+      a = a + w.x * (vm + vp);
+      b = b + w.y * (vm - vp);
+    }
+    const C2<T> rb = rot(b, dir);
+    d[kk * ns] = a + rb;
+    d[(R - kk) * ns] = a - rb;
   }
 }
 
@@ -482,8 +503,47 @@ __device__ __forceinline__ C2<T> pre_elem(int mode, const T *x, int64_t es, int6
       }
       break;
     }
+    case kPreRealPairs:
+      if (2 * j < n) v = C2<T>{x[2 * j * es], x[(2 * j + 1) * es]};
+      break;
+    case kPreHalfSpectrum: {
+      // z_j = Σ_k [A_k + i e^{2πik/n} D_k] e^{2πikj/(n/2)} with A = X_k + conj X_{h-k},
+      // D = X_k - conj X_{h-k}: the inverse FFT of length h = n/2 gives
+      // y_2j + i y_2j+1.  Imaginary parts of bins 0 and h are ignored.
+      const int64_t h = n / 2;
+      C2<T> xj = {x[j * es], x[j * es + 1]}, xm = {x[(h - j) * es], x[(h - j) * es + 1]};
+      if (conj) {
+        xj.y = -xj.y;
+        xm.y = -xm.y;
+      }
+      if (j == 0) {
+        xj.y = T(0);
+        xm.y = T(0);
+      }
+      const C2<T> A = xj + conjc(xm);
+      const C2<T> B = cmul(xj - conjc(xm), expipi<T>(2 * j, n));
+      v = C2<T>{A.x - B.y, A.y + B.x};
+      break;
+    }
   }
   return v;
+}
+
+// r2c bin k (0..n/2) of a real line of even length n from the length-n/2
+// spectrum Z of its pairs z_j = x_2j + i x_2j+1 (get(q) = Z_q):
+// X_k = (Z_k + conj Z_{h-k}) / 2 + e^{-2πik/n} (Z_k - conj Z_{h-k}) / 2i.
+template <typename T, typename G>
+__device__ __forceinline__ C2<T> half_spectrum_bin(G &&get, int64_t n, int64_t k) {
+  const int64_t h = n / 2;
+  if (k == 0 || k == h) {
+    const C2<T> z = get(0);
+    return C2<T>{k == 0 ? z.x + z.y : z.x - z.y, T(0)};
+  }
+  const C2<T> zk = get(k), zm = conjc(get(h - k));
+  const C2<T> fe = T(0.5) * (zk + zm);
+  const C2<T> dd = zk - zm;
+  const C2<T> fo = C2<T>{T(0.5) * dd.y, T(-0.5) * dd.x};
+  return fe + cmul(expipi<T>(-2 * k, n), fo);
 }
 
 // Real output element k from the transformed work line (get(q) = bin q)
@@ -536,6 +596,9 @@ __device__ __forceinline__ T post_elem(int mode, G &&get, int64_t n, int64_t k, 
       if (k == 0) r = get(0).x;
       else if (k % 2) r = get((k + 1) / 2).x;
       else r = get(k / 2).y;
+      break;
+    case kPostRealPairs:
+      r = (k % 2) ? get(k / 2).y : get(k / 2).x;
       break;
   }
   return r;

@@ -217,13 +217,16 @@ void r2c_axis(Ctx &c, const int64_t *shape, int nd, int ax, const int64_t *si,
     launch_real_fused(*rp, kR2C, g, !forward, fct, c.s);
     return;
   }
-  void *work = complex_work(c, g.nlines, n);
+  // even n: FFT of the n/2 pairs, then the split twiddle; odd n: full length
+  const int64_t m = rp->half ? n / 2 : n;
+  void *work = complex_work(c, g.nlines, m);
   RealPrePost rr;
-  rr.n = rr.m = n;
-  launch_pre(kPreRealAsComplex, rr, g, c.dbl, work, c.s);
-  c2c_work(c, work, g.nlines, n, true);
+  rr.n = n;
+  rr.m = m;
+  launch_pre(rp->half ? kPreRealPairs : kPreRealAsComplex, rr, g, c.dbl, work, c.s);
+  c2c_work(c, work, g.nlines, m, true);
   rr.conj = !forward;
-  launch_post(kPostComplexHalf, rr, g, c.dbl, work, fct, c.s);
+  launch_post(rp->half ? kPostHalfSpectrum : kPostComplexHalf, rr, g, c.dbl, work, fct, c.s);
 }
 
 // n/2+1 complex -> real along ax; shape is the real shape
@@ -236,14 +239,16 @@ void c2r_axis(Ctx &c, const int64_t *shape, int nd, int ax, const int64_t *si,
     launch_real_fused(*rp, kC2R, g, forward, fct, c.s);
     return;
   }
-  void *work = complex_work(c, g.nlines, n);
+  const int64_t m = rp->half ? n / 2 : n;
+  void *work = complex_work(c, g.nlines, m);
   RealPrePost rr;
-  rr.n = rr.m = n;
+  rr.n = n;
+  rr.m = m;
   rr.conj = forward;
-  launch_pre(kPreHermitian, rr, g, c.dbl, work, c.s);
-  c2c_work(c, work, g.nlines, n, false);
+  launch_pre(rp->half ? kPreHalfSpectrum : kPreHermitian, rr, g, c.dbl, work, c.s);
+  c2c_work(c, work, g.nlines, m, false);
   rr.conj = false;
-  launch_post(kPostReal, rr, g, c.dbl, work, fct, c.s);
+  launch_post(rp->half ? kPostRealPairs : kPostReal, rr, g, c.dbl, work, fct, c.s);
 }
 
 void r2c_nd(Ctx &c, const int64_t *shape, int nd, const int64_t *si, const int64_t *so,

@@ -515,8 +515,10 @@ __global__ void __launch_bounds__(kThreads) k_post(const PrePostArgs<T> a) {
     const int64_t n = a.n;
     for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < a.count;
          k += int64_t(gridDim.x) * blockDim.x) {
-      if (a.mode == kPostComplexHalf) {
-        C2<T> v = t[k];
+      if (a.mode == kPostComplexHalf || a.mode == kPostHalfSpectrum) {
+        C2<T> v = a.mode == kPostComplexHalf
+                      ? t[k]
+                      : half_spectrum_bin<T>([&](int64_t q) { return t[q]; }, n, k);
         if (a.conj) v.y = -v.y;
         y[k * a.es] = a.fct * v.x;
         y[k * a.es + 1] = a.fct * v.y;
@@ -582,7 +584,7 @@ void launch_post(PostMode mode, const RealPrePost &rp, const LineSet &g, bool db
     a.tmp = static_cast<C2<T> *>(const_cast<void *>(tmp_complex));
     a.mode = mode;
     a.fct = T(fct);
-    a.count = mode == kPostComplexHalf ? rp.n / 2 + 1 : rp.n;
+    a.count = (mode == kPostComplexHalf || mode == kPostHalfSpectrum) ? rp.n / 2 + 1 : rp.n;
     dim3 grid;
     grid_for(a.count, g.nlines, grid);
     hipLaunchKernelGGL(k_post<T>, grid, dim3(kThreads), 0, s, a);

@@ -429,8 +429,13 @@ static std::shared_ptr<RealPlan> build_real(uint64_t n, bool dbl, size_t budget)
   p->dbl = dbl;
   const uint64_t m = (n % 2 == 0) ? n / 2 : n;
   uint64_t bn2;
-  // odd lengths that the reference sends to Bluestein stay on the generic path
-  const bool blue = (n % 2 == 1) && n > 1 && plan_bluestein(n, false, &bn2);
+  // The fused pass runs the complex FFT of m as one LDS-resident mixed-radix
+  // plan.  When the Bluestein rule picks Bluestein for m (a large prime
+  // factor, whose generic radix stage would cost O(m p) in one workgroup), or
+  // m does not fit LDS, even lengths run the FFT of the n/2 pairs through the
+  // complex plan of n/2 with the split twiddle in the pre/post kernels, and
+  // odd lengths the complex plan of n.
+  const bool blue = m > 1 && plan_bluestein(m, false, &bn2);
   if (n >= 2 && !blue && m <= direct_max(dbl, 1, budget)) {
     p->fused = true;
     make_direct(p->d, m, dbl);
@@ -439,6 +444,8 @@ static std::shared_ptr<RealPlan> build_real(uint64_t n, bool dbl, size_t budget)
       for (uint64_t k = 0; k <= n / 2; ++k) rtw[k] = wfwd(k, n);
       upload_any(p->rtw, rtw, dbl);
     }
+  } else {
+    p->half = n >= 2 && n % 2 == 0;
   }
   return p;
 }

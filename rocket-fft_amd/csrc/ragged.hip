@@ -54,9 +54,11 @@ __device__ void post_line(const LineArgs<T> &a) {
   const C2<T> *t = reinterpret_cast<const C2<T> *>(a.in);
   const int mode = (a.sub >> 8) & 0xff;
   const int64_t n = a.bign;
-  if (mode == kPostComplexHalf) {
+  if (mode == kPostComplexHalf || mode == kPostHalfSpectrum) {
     for (int64_t k = threadIdx.x; k <= n / 2; k += blockDim.x) {
-      C2<T> v = t[k];
+      C2<T> v = mode == kPostComplexHalf
+                    ? t[k]
+                    : half_spectrum_bin<T>([&](int64_t q) { return t[q]; }, n, k);
       if (a.sub & R2R_CONJ) v.y = -v.y;
       a.out[k * a.eso] = a.fct * v.x;
       a.out[k * a.eso + 1] = a.fct * v.y;
@@ -172,10 +174,11 @@ std::unique_ptr<Built> build(const std::vector<rfh_job_t> &jobs, bool dbl, size_
       arena = align_up(arena + c2c_tmp_bytes(*cplan[i], one_line(nullptr, nullptr, 2, 2)), 256);
     } else {
       rplan[i] = get_real_plan(j.n, dbl, budget);
-      if (!rplan[i]->fused) {  // pre -> c2c(n) -> post through a work line
-        cplan[i] = get_c2c_plan(j.n, dbl, budget);
+      if (!rplan[i]->fused) {  // pre -> c2c(m) -> post through a work line of m
+        const uint64_t m = rplan[i]->half ? j.n / 2 : j.n;
+        cplan[i] = get_c2c_plan(m, dbl, budget);
         work_off[i] = arena;
-        arena = align_up(arena + size_t(j.n) * ce, 256);
+        arena = align_up(arena + size_t(m) * ce, 256);
         tmp_off[i] = arena;
         arena = align_up(arena + c2c_tmp_bytes(*cplan[i], one_line(nullptr, nullptr, 2, 2)), 256);
       }
@@ -208,7 +211,12 @@ std::unique_ptr<Built> build(const std::vector<rfh_job_t> &jobs, bool dbl, size_
     pre.d = &cplan[i]->main;  // only desc.n (= m) is read
     pre.op = OP_PRE;
     pre.bign = int64_t(j.n);
-    pre.sub = int(r2c ? kPreRealAsComplex : kPreHermitian) | (!r2c && j.forward ? R2R_CONJ : 0);
+    const bool half = rplan[i]->half;
+    const int pre_mode = r2c ? (half ? kPreRealPairs : kPreRealAsComplex)
+                             : (half ? kPreHalfSpectrum : kPreHermitian);
+    const int post_mode = r2c ? (half ? kPostHalfSpectrum : kPostComplexHalf)
+                              : (half ? kPostRealPairs : kPostReal);
+    pre.sub = pre_mode | (!r2c && j.forward ? R2R_CONJ : 0);
     per[i].push_back(PassLaunch{pre, one_line(j.in, work, r2c ? 1 : 2, 2)});
     for (auto &pl : describe_c2c(*cplan[i], one_line(work, work, 2, 2), r2c ? -1 : +1, 1.0,
                                  ar + tmp_off[i]))
@@ -218,7 +226,7 @@ std::unique_ptr<Built> build(const std::vector<rfh_job_t> &jobs, bool dbl, size_
     post.op = OP_POST;
     post.bign = int64_t(j.n);
     post.fct = j.fct;
-    post.sub = (int(r2c ? kPostComplexHalf : kPostReal) << 8) | (r2c && !j.forward ? R2R_CONJ : 0);
+    post.sub = (post_mode << 8) | (r2c && !j.forward ? R2R_CONJ : 0);
     per[i].push_back(PassLaunch{post, one_line(work, j.out, 2, r2c ? 2 : 1)});
   }
   // pass 3: device tables
@@ -233,7 +241,7 @@ std::unique_ptr<Built> build(const std::vector<rfh_job_t> &jobs, bool dbl, size_
     for (const PassLaunch &pl : per[i]) {
       LineArgs<T> a = make_line_args<T>(pl.ps, pl.g);
       if (pl.ps.op == OP_PRE || pl.ps.op == OP_POST) {
-        a.f.n = int(cplan[i]->n);  // work-line length m (= n)
+        a.f.n = int(cplan[i]->n);  // work-line length m
         a.L = 1;
       } else {
         lds_need = std::max<int>(lds_need, int(2 * a.L * a.ld * ce));

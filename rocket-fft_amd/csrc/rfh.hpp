@@ -100,7 +100,8 @@ struct C2CPlan {
 struct RealPlan {
   uint64_t n = 0;
   bool dbl = false;
-  bool fused = false;
+  bool fused = false;  // one LDS pass (d is the complex FFT of n/2 or n)
+  bool half = false;   // even n otherwise: pairs -> c2c plan of n/2 -> twiddle
   Direct d;
   DevBuf rtw;
 };
@@ -167,6 +168,8 @@ enum PreMode {
   kPreDst3,
   kPreType4,
   kPreHalfcomplex,        // FFTPACK halfcomplex -> Hermitian spectrum
+  kPreRealPairs,          // r2c, even n: a_j = x_2j + i x_2j+1 (j < n/2)
+  kPreHalfSpectrum,       // c2r, even n: work line of the length-n/2 inverse FFT
 };
 enum PostMode {
   kPostComplexHalf = 0,   // bins 0..n/2 (r2c)
@@ -181,6 +184,8 @@ enum PostMode {
   kPostDst4,
   kPostHartley,           // Re + Im
   kPostPackHalfcomplex,   // bins -> FFTPACK halfcomplex
+  kPostHalfSpectrum,      // r2c, even n: bins 0..n/2 from the length-n/2 spectrum
+  kPostRealPairs,         // c2r, even n: y_2j = Re z_j, y_2j+1 = Im z_j
 };
 struct RealPrePost {
   int64_t n = 0;      // logical line length (real)

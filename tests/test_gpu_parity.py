@@ -58,8 +58,8 @@ def test_bluestein_two_pass_pow2_columns(dtype):
     """n = 50174 = 2 * 25087: Bluestein length 100352 = 256 * 392, so the
     column pass is a power-of-two length and runs on the specialised line
     kernel, which must apply the chirp (and the zero padding from m = n on)
-    while it loads.  c2c both ways, and r2c / c2r, whose unfused form runs
-    the same c2c of length n."""
+    while it loads.  c2c both ways; r2c / c2r of the same n run the half
+    length 25087 (a Bluestein prime) between the pair pre/post kernels."""
     n = 50174
     dbl = dtype == np.complex128
     kind, _, n1, n2 = R.gpu_schedule(n, dbl)
@@ -87,8 +87,10 @@ def test_c2c_axes(shape, axes, dtype):
     check("c2c", x, x, np.array(axes), False, 1.0, inplace=True)
 
 
+# 3086 = 2 * 1543, 18134 = 2 * 9067 (half length a Bluestein prime: pairs ->
+# c2c of n/2 -> split twiddle), 40000 and 65536 (half length beyond one LDS line)
 @pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 9, 15, 16, 30, 64, 101, 128, 255, 256,
-                               512, 1000, 4097, 4099, 8192])
+                               512, 1000, 4097, 4099, 8192, 3086, 18134, 40000, 65536])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_r2c_c2r_lengths(n, dtype):
     rs = np.random.RandomState(n + 3)
@@ -102,7 +104,8 @@ def test_r2c_c2r_lengths(n, dtype):
 
 
 @pytest.mark.parametrize("shape,axes", [((8, 6, 10), (0, 1, 2)), ((8, 6, 10), (2, 0)),
-                                        ((9, 7), (0, 1)), ((12, 5), (0,)), ((4, 4, 4), (1,))])
+                                        ((9, 7), (0, 1)), ((12, 5), (0,)), ((4, 4, 4), (1,)),
+                                        ((3086, 3), (0,)), ((3, 3086), (1, 0))])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_r2c_c2r_axes(shape, axes, dtype):
     rs = np.random.RandomState(2)
