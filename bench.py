@@ -123,6 +123,39 @@ def extra_configs(torch, R, stream, reps):
     return out
 
 
+def lengths_config(torch, R, stream, reps):
+    """Mixed-radix and prime lengths, batch 256, complex64 and complex128:
+    forward c2c timed (graph replays), forward error against numpy's FP64
+    transform on the host, and the forward -> inverse (fct = 1/N) round trip."""
+    out = {}
+    rs = np.random.RandomState(3)
+    ax = np.array([1])
+    for n in (1000, 1536, 2187, 4913, 4099, 65537):
+        x64 = rs.standard_normal((256, n)) + 1j * rs.standard_normal((256, n))
+        want = np.fft.fft(x64, axis=1)
+        for dt, npdt, name in ((torch.complex64, np.complex64, "c64"),
+                               (torch.complex128, np.complex128, "c128")):
+            xh = x64.astype(npdt)
+            a = torch.from_numpy(xh).cuda()
+            b = torch.empty_like(a)
+            c = torch.empty_like(a)
+            with torch.cuda.stream(stream):
+                R.c2c(a, b, ax, True, 1.0, 1)
+                R.c2c(b, c, ax, False, 1.0 / n, 1)
+            torch.cuda.synchronize()
+            ref = np.fft.fft(xh.astype(np.complex128), axis=1) if npdt == np.complex64 else want
+            fwd_err = float(np.linalg.norm(b.cpu().numpy() - ref) / np.linalg.norm(ref))
+            rt_err = float(np.linalg.norm(c.cpu().numpy() - xh) / np.linalg.norm(xh))
+            ms, _ = graph_time(torch, lambda: R.c2c(a, b, ax, True, 1.0, 1), stream, reps)
+            nbytes = 2 * a.numel() * a.element_size()
+            out[f"{name}_n{n}"] = {"ms": round(ms, 4), "transforms_per_s": round(256 / ms * 1e3),
+                                   "GB/s": round(nbytes / ms / 1e6, 1),
+                                   "hbm_frac": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 3),
+                                   "rel_l2_fwd_vs_fp64": fwd_err, "rel_l2_roundtrip": rt_err}
+            del a, b, c
+    return out
+
+
 def probe(torch, R):
     """Calibration copy (reads and writes BYTES_C2C_PASS/2 each) then the
     dominant kernel, three launches per strided axis, no graph."""
@@ -352,6 +385,7 @@ def main():
         res["configs"] = extra_configs(torch, R, stream, 10)
         res["configs"]["ragged_4096_c64_r2c"] = ragged_config(torch, R, stream)
         res["configs"]["precision_c64_16384x4096"] = precision_config(torch, R, stream)
+        res["configs"]["lengths_batch256"] = lengths_config(torch, R, stream, 10)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(x_h, d_h, args.cpu_seconds)
     print(json.dumps(res))

@@ -163,11 +163,14 @@ int rfh_cfftp_factors(uint64_t n, uint64_t *out, int cap);
 int rfh_rfftp_factors(uint64_t n, uint64_t *out, int cap);
 int rfh_plan_choice(uint64_t n, bool real, uint64_t *n2);
 
-/* GPU execution schedule chosen for a complex transform of length n
- * (double_precision selects complex128): kind 0 = single pass in LDS,
- * 1 = Bluestein in LDS, 2 = two-pass (four-step), 3 = Bluestein two-pass;
- * radices of the (first) LDS pass in execution order; n1, n2 = the
- * two-pass split (0 when single pass).  Returns the number of radices. */
+/* GPU execution schedule chosen for a complex transform of length n over
+ * contiguous lines (double_precision selects complex128): kind 0 = single
+ * pass in LDS, 1 = Bluestein in LDS, 2 = two-pass (four-step), 3 = Bluestein
+ * two-pass, 4 = multi-pass (a column pass of n1, then the rows of n2 = n / n1
+ * by their own two- or multi-pass plan), 5 = Bluestein around multi-pass
+ * transforms of its padded length; radices of the (first) LDS pass in
+ * execution order; n1, n2 = the split (0 when single pass).  Returns the
+ * number of radices. */
 int rfh_gpu_schedule(uint64_t n, bool double_precision, int *kind, int *radices, int cap,
                      uint64_t *n1, uint64_t *n2);
 

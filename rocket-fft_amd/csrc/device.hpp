@@ -336,6 +336,10 @@ enum LineOp {
   OP_C2R_EVEN = 6,
   OP_C2R_ODD = 7,
   OP_R2R = 8,     // real line -> pre_elem -> FFT -> post_elem (DCT/DST, Hartley, FFTPACK)
+  // elementwise passes of the multi-pass Bluestein form (no FFT; count = other)
+  OP_BLUE_IN = 11,   // work[m] = x[m] * conj(chirp[m]) for m < n, zero up to bn2
+  OP_BLUE_MUL = 12,  // work[m] *= kernel[m]
+  OP_BLUE_OUT = 13,  // y[m] = fct * work[m] * conj(chirp[m]) for m < n
 };
 // OP_R2R packs its modes into LineArgs::sub
 constexpr int R2R_ORTHO = 1 << 16, R2R_CONJ = 1 << 17, R2R_FLIP = 1 << 18;
@@ -355,6 +359,7 @@ struct LineArgs {
   const T *in;
   T *out;
   int L, log2L, ld, nin, nout, lfast_in, lfast_out;
+  int auxdim;    // batch dim whose index is a line's aux (four-step row / column)
   FftDesc f;
   const C2<T> *tw, *gen;
   T dir, fct;
@@ -625,7 +630,7 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
       rem = q;
       oi += idx * a.bsi[d];
       oo += idx * a.bso[d];
-      if (d == a.nb - 1) aux = int(idx);
+      if (d == a.auxdim) aux = int(idx);
     }
     s_oi[l] = oi;
     s_oo[l] = oo;

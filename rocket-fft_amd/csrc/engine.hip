@@ -181,8 +181,14 @@ struct Ctx {
 void c2c_axis(Ctx &c, const int64_t *shape, int nd, int ax, const int64_t *si,
               const int64_t *so, const void *in, void *out, bool forward, double fct) {
   const uint64_t n = uint64_t(shape[ax]);
-  auto plan = get_c2c_plan(n, c.dbl);
   LineSet g = lines_for_axis(shape, nd, ax, si, so, in, out);
+  // lines side by side in memory (a batch dim closer than the elements of
+  // a line, on either side) want the interleaved plan
+  bool inter = false;
+  for (int d = 0; d < g.nb; ++d)
+    inter = inter || std::llabs(g.bsi[d]) < std::llabs(g.esi) ||
+            std::llabs(g.bso[d]) < std::llabs(g.eso);
+  auto plan = get_c2c_plan(n, c.dbl, 0, inter);
   void *tmp = c.ws->get(kSlotTwoPass, c2c_tmp_bytes(*plan, g));
   launch_c2c(*plan, g, forward ? -1 : +1, fct, c.s, tmp);
 }

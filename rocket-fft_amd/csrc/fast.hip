@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(TPL *LPB) k_fast(const LineArgs<T> a) {
       rem = q;
       oi += idx * a.bsi[d];
       oo += idx * a.bso[d];
-      if (d == a.nb - 1) aux = int(idx);
+      if (d == a.auxdim) aux = int(idx);
     }
   }
   c.gin = a.in + oi;
@@ -321,23 +321,54 @@ bool go_map(const LineArgs<T> &a, bool lf_in, bool lf_out, bool vin, bool vout,
   return false;  // unaligned layouts use the generic kernel
 }
 
-// threads per line and lines per block for length N
+// Thread and line shape of the specialised kernel for length N.
+template <typename T, int N>
+struct FastShape {
+  static constexpr int E = 16;  // elements per thread (largest radix)
+  static constexpr int TPL = N >= E ? N / E : 1;
+  // contiguous lines: 256-thread blocks (one line per block from N = 4096)
+  static constexpr int LPB_C = TPL >= 256 ? 1 : 256 / TPL;
+  // interleaved lines: 128-byte row segments when the block (threads and
+  // LDS) allows it, else at least 8 adjacent lines
+  static constexpr int SEG = 128 / int(sizeof(C2<T>));
+  static constexpr int LPB_8 = LPB_C >= 8 ? LPB_C : 8;
+  static constexpr int LPB_W = LPB_8 >= SEG ? LPB_8 : SEG;
+  static constexpr bool fits(int lpb) {
+    return TPL * lpb <= 1024 && size_t(lpb) * pitch_of<T, N>() * sizeof(C2<T>) <= 152 * 1024;
+  }
+  static constexpr int LPB_I = fits(LPB_W) ? LPB_W : LPB_8;
+  static constexpr bool interleaved_ok = fits(LPB_I);
+};
+
 template <typename T, int N, int OP, typename SC>
 bool go_len(const LineArgs<T> &a, bool lf_in, bool lf_out, bool vin, bool vout,
             hipStream_t s) {
-  constexpr int E = 16;  // elements per thread (largest radix)
-  constexpr int TPL = N >= E ? N / E : 1;
-  // contiguous lines: 256-thread blocks (one line per block from N = 4096)
-  constexpr int LPB_C = TPL >= 256 ? 1 : 256 / TPL;
-  // interleaved lines: at least 8 adjacent lines per block (64-byte rows)
-  constexpr int LPB_I = LPB_C >= 8 ? LPB_C : 8;
+  using F = FastShape<T, N>;
   if (lf_in || lf_out) {
-    if constexpr (TPL * LPB_I <= 1024 &&
-                  size_t(LPB_I) * pitch_of<T, N>() * sizeof(C2<T>) <= 152 * 1024)
-      return go_map<T, N, TPL, LPB_I, OP, SC>(a, lf_in, lf_out, vin, vout, s);
+    if constexpr (F::interleaved_ok)
+      return go_map<T, N, F::TPL, F::LPB_I, OP, SC>(a, lf_in, lf_out, vin, vout, s);
     return false;
   }
-  return go_map<T, N, TPL, LPB_C, OP, SC>(a, lf_in, lf_out, vin, vout, s);
+  return go_map<T, N, F::TPL, F::LPB_C, OP, SC>(a, lf_in, lf_out, vin, vout, s);
+}
+
+template <typename T>
+bool interleaved_ok_t(uint64_t n) {
+  switch (n) {
+    case 2: return FastShape<T, 2>::interleaved_ok;
+    case 4: return FastShape<T, 4>::interleaved_ok;
+    case 8: return FastShape<T, 8>::interleaved_ok;
+    case 16: return FastShape<T, 16>::interleaved_ok;
+    case 32: return FastShape<T, 32>::interleaved_ok;
+    case 64: return FastShape<T, 64>::interleaved_ok;
+    case 128: return FastShape<T, 128>::interleaved_ok;
+    case 256: return FastShape<T, 256>::interleaved_ok;
+    case 512: return FastShape<T, 512>::interleaved_ok;
+    case 1024: return FastShape<T, 1024>::interleaved_ok;
+    case 2048: return FastShape<T, 2048>::interleaved_ok;
+    case 4096: return FastShape<T, 4096>::interleaved_ok;
+    default: return false;
+  }
 }
 
 template <typename T, int OP>
@@ -363,6 +394,10 @@ bool go_op(const LineArgs<T> &a, bool lf_in, bool lf_out, bool vin, bool vout,
 }  // namespace
 
 bool g_fast_enabled = true;
+
+bool fast_interleaved_ok(uint64_t n, bool dbl) {
+  return dbl ? interleaved_ok_t<double>(n) : interleaved_ok_t<float>(n);
+}
 
 // Launches the specialised kernel for (op, length, layout) when one exists.
 template <typename T>

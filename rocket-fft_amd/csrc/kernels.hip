@@ -50,8 +50,10 @@ int ilog2(int v) {
 // lines per group: the largest power of two whose two LDS buffers fit
 // `budget` (and are not wasted on missing lines)
 int choose_lines(int ld, size_t elem, int64_t nlines, bool interleaved) {
-  const size_t budget = std::min<size_t>(interleaved ? 96 * 1024 : 48 * 1024,
-                                         lds_limit() - 8192);
+  // interleaved lines want as many adjacent lines as LDS holds (row
+  // segments); contiguous lines leave room for several groups per CU
+  const size_t budget = interleaved ? lds_limit() - 8192
+                                    : std::min<size_t>(48 * 1024, lds_limit() - 8192);
   int L = 1;
   while (L * 2 <= kMaxLinesPerGroup && size_t(2) * (L * 2) * ld * elem <= budget &&
          int64_t(L) < nlines)
@@ -134,6 +136,7 @@ LineArgs<T> make_line_args(const PassSpec &ps, const LineSet &g) {
   }
   a.esi = g.esi;
   a.eso = g.eso;
+  a.auxdim = g.auxdim >= 0 ? g.auxdim : g.nb - 1;
   a.in = static_cast<const T *>(g.in);
   a.out = static_cast<T *>(g.out);
   a.f = ps.d->desc;
@@ -165,9 +168,55 @@ template LineArgs<double> make_line_args<double>(const PassSpec &, const LineSet
 
 namespace {
 
+// The elementwise Bluestein passes (chirp in, kernel product, chirp out) of
+// the multi-pass form, one grid row per line.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_blue_elem(const LineArgs<T> a, int op) {
+  const bool cj = (a.sub & SUB_CONJ) != 0;
+  for (int64_t b = blockIdx.y; b < a.nlines; b += gridDim.y) {
+    int64_t rem = b, oi = 0, oo = 0;
+    for (int d = a.nb - 1; d >= 0; --d) {
+      const int64_t q = rem / a.bdim[d];
+      const int64_t idx = rem - q * a.bdim[d];
+      rem = q;
+      oi += idx * a.bsi[d];
+      oo += idx * a.bso[d];
+    }
+    const T *x = a.in + oi;
+    T *y = a.out + oo;
+    for (int64_t m = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; m < a.other;
+         m += int64_t(gridDim.x) * blockDim.x) {
+      C2<T> v = {T(0), T(0)};
+      if (op == OP_BLUE_IN) {
+        if (m < a.bign) {
+          const C2<T> c = a.t0[m];
+          v = cmul(ldc<false>(x + m * a.esi), cj ? c : conjc(c));
+        }
+      } else if (op == OP_BLUE_MUL) {
+        const C2<T> w = a.t1[m];
+        v = cmul(ldc<false>(x + m * a.esi), cj ? conjc(w) : w);
+      } else {
+        const C2<T> c = a.t0[m];
+        v = a.fct * cmul(ldc<false>(x + m * a.esi), cj ? c : conjc(c));
+      }
+      stc<false>(y + m * a.eso, v);
+    }
+  }
+}
+
 template <typename T>
 void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
   if (g.nlines <= 0) return;
+  if (ps.op == OP_BLUE_IN || ps.op == OP_BLUE_MUL || ps.op == OP_BLUE_OUT) {
+    LineArgs<T> a = make_line_args<T>(ps, g);
+    dim3 grid;
+    grid.x = unsigned(std::max<int64_t>(1, std::min<int64_t>((ps.other + kThreads - 1) / kThreads,
+                                                             1024)));
+    grid.y = unsigned(std::max<int64_t>(1, std::min<int64_t>(g.nlines, 65535)));
+    hipLaunchKernelGGL(k_blue_elem<T>, grid, dim3(kThreads), 0, s, a, ps.op);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
   LineArgs<T> a = make_line_args<T>(ps, g);
   const size_t elem = 2 * sizeof(T);
   const size_t lds = size_t(2) * a.L * a.ld * elem;
@@ -204,55 +253,6 @@ void run_pass_any(bool dbl, const PassSpec &ps, const LineSet &g, hipStream_t s)
     run_pass<float>(ps, g, s);
 }
 
-// Line sets of the two passes of a four-step transform of length N = n1*n2
-// over the lines of g.  Pass A: line (b, c) = column c of line b, elements
-// n1 apart in units of the column stride; pass B: row r of the n1 x n2
-// intermediate, stored to position r + n1*k of the output line.
-LineSet pass_a_lines(const LineSet &g, uint64_t n1, uint64_t n2, const void *in,
-                     int64_t esi_line, void *tmp) {
-  LineSet a{};
-  a.in = in;
-  a.out = tmp;
-  a.nlines = g.nlines * int64_t(n2);
-  a.nb = g.nb + 1;
-  if (a.nb > kMaxBatchDims) RFH_THROW(3, "too many batch dimensions");
-  int64_t tstride = int64_t(2 * n1 * n2);
-  for (int d = g.nb - 1; d >= 0; --d) {
-    a.bdim[d] = g.bdim[d];
-    a.bsi[d] = g.bsi[d];
-    a.bso[d] = tstride;
-    tstride *= g.bdim[d];
-  }
-  a.bdim[g.nb] = int64_t(n2);
-  a.bsi[g.nb] = esi_line;
-  a.bso[g.nb] = 2;
-  a.esi = esi_line * int64_t(n2);
-  a.eso = 2 * int64_t(n2);
-  return a;
-}
-
-LineSet pass_b_lines(const LineSet &g, uint64_t n1, uint64_t n2, const void *tmp, void *out,
-                     int64_t eso_line, const int64_t *bso) {
-  LineSet b{};
-  b.in = tmp;
-  b.out = out;
-  b.nlines = g.nlines * int64_t(n1);
-  b.nb = g.nb + 1;
-  int64_t tstride = int64_t(2 * n1 * n2);
-  for (int d = g.nb - 1; d >= 0; --d) {
-    b.bdim[d] = g.bdim[d];
-    b.bsi[d] = tstride;
-    b.bso[d] = bso[d];
-    tstride *= g.bdim[d];
-  }
-  b.bdim[g.nb] = int64_t(n1);
-  b.bsi[g.nb] = 2 * int64_t(n2);
-  b.bso[g.nb] = eso_line;
-  b.esi = 2;
-  b.eso = eso_line * int64_t(n1);
-  return b;
-}
-
 // contiguous complex line set with g's batch shape, lines of length len
 void contiguous_like(const LineSet &g, uint64_t len, int64_t *bs) {
   int64_t st = int64_t(2 * len);
@@ -260,6 +260,151 @@ void contiguous_like(const LineSet &g, uint64_t len, int64_t *bs) {
     bs[d] = st;
     st *= g.bdim[d];
   }
+}
+
+// Orders the batch dims by decreasing |stride| on the input (by_out false)
+// or output side, so that the lines of one group are the closest in memory
+// on that side; auxdim follows its dim.
+void order_dims(LineSet &g, bool by_out) {
+  int idx[kMaxBatchDims];
+  for (int d = 0; d < g.nb; ++d) idx[d] = d;
+  const int64_t *st = by_out ? g.bso : g.bsi;
+  std::stable_sort(idx, idx + g.nb,
+                   [&](int a, int b) { return std::llabs(st[a]) > std::llabs(st[b]); });
+  LineSet o = g;
+  o.auxdim = -1;
+  for (int d = 0; d < g.nb; ++d) {
+    o.bdim[d] = g.bdim[idx[d]];
+    o.bsi[d] = g.bsi[idx[d]];
+    o.bso[d] = g.bso[idx[d]];
+    if (idx[d] == (g.auxdim >= 0 ? g.auxdim : g.nb - 1)) o.auxdim = d;
+  }
+  g = o;
+}
+
+// Lines interleaved on the input side: a batch dim is closer in memory
+// than the elements of one line.
+bool input_interleaved(const LineSet &g) {
+  for (int d = 0; d < g.nb; ++d)
+    if (std::llabs(g.bsi[d]) < std::llabs(g.esi)) return true;
+  return false;
+}
+
+// Layout of a four-step intermediate of g's lines, each an n1 x n2 matrix
+// (rows k1, columns c): line-major with rows of n2 (contiguous lines), or
+// line-fastest (interleaved lines), so that the passes over it keep the
+// same adjacent-line order as the user's arrays.
+struct Tmp4 {
+  void *p = nullptr;
+  int64_t sb[kMaxBatchDims] = {};
+  int64_t srow = 0, scol = 0;
+};
+
+Tmp4 tmp4(const LineSet &g, uint64_t n1, uint64_t n2, void *p, bool inter) {
+  Tmp4 t;
+  t.p = p;
+  if (inter) {
+    contiguous_like(g, 1, t.sb);
+    t.scol = 2 * g.nlines;
+    t.srow = t.scol * int64_t(n2);
+  } else {
+    contiguous_like(g, n1 * n2, t.sb);
+    t.scol = 2;
+    t.srow = 2 * int64_t(n2);
+  }
+  return t;
+}
+
+// Natural-order work lines of length len in the same two layouts.
+struct Work {
+  int64_t sb[kMaxBatchDims] = {};
+  int64_t e = 0;
+};
+
+Work work_layout(const LineSet &g, uint64_t len, bool inter) {
+  Work w;
+  if (inter) {
+    contiguous_like(g, 1, w.sb);
+    w.e = 2 * g.nlines;
+  } else {
+    contiguous_like(g, len, w.sb);
+    w.e = 2;
+  }
+  return w;
+}
+
+// Pass A of a four-step transform N = n1 * n2 over the lines of g: line
+// (b, c) is column c of line b (n1 elements, n2 apart), written to row-major
+// position (k1, c) of the intermediate t.  aux = c.
+LineSet pass_a(const LineSet &g, uint64_t n1, uint64_t n2, const void *in, int64_t esi_line,
+               const int64_t *inbs, const Tmp4 &t) {
+  LineSet a{};
+  a.in = in;
+  a.out = t.p;
+  a.nlines = g.nlines * int64_t(n2);
+  a.nb = g.nb + 1;
+  if (a.nb > kMaxBatchDims) RFH_THROW(3, "too many batch dimensions");
+  for (int d = 0; d < g.nb; ++d) {
+    a.bdim[d] = g.bdim[d];
+    a.bsi[d] = inbs[d];
+    a.bso[d] = t.sb[d];
+  }
+  a.bdim[g.nb] = int64_t(n2);
+  a.bsi[g.nb] = esi_line;
+  a.bso[g.nb] = t.scol;
+  a.esi = esi_line * int64_t(n2);
+  a.eso = t.srow;
+  a.auxdim = g.nb;
+  order_dims(a, false);
+  return a;
+}
+
+// Pass B: line (b, k1) is row k1 of the intermediate (n2 elements), its
+// bin k2 stored to position k1 + n1 * k2 of the output line.  aux = k1.
+LineSet pass_b(const LineSet &g, uint64_t n1, uint64_t n2, const Tmp4 &t, void *out,
+               int64_t eso_line, const int64_t *outbs) {
+  LineSet b{};
+  b.in = t.p;
+  b.out = out;
+  b.nlines = g.nlines * int64_t(n1);
+  b.nb = g.nb + 1;
+  if (b.nb > kMaxBatchDims) RFH_THROW(3, "too many batch dimensions");
+  for (int d = 0; d < g.nb; ++d) {
+    b.bdim[d] = g.bdim[d];
+    b.bsi[d] = t.sb[d];
+    b.bso[d] = outbs[d];
+  }
+  b.bdim[g.nb] = int64_t(n1);
+  b.bsi[g.nb] = t.srow;
+  b.bso[g.nb] = eso_line;
+  b.esi = t.scol;
+  b.eso = eso_line * int64_t(n1);
+  b.auxdim = g.nb;
+  order_dims(b, true);
+  return b;
+}
+
+// Rows of a multi-pass intermediate t (lines (b, k1) of length M) as the
+// input of the sub-plan, output to position k1 + n1 * k' of g's lines.
+LineSet multi_rows(const LineSet &g, uint64_t n1, const Tmp4 &t) {
+  LineSet r{};
+  r.in = t.p;
+  r.out = g.out;
+  r.nlines = g.nlines * int64_t(n1);
+  r.nb = g.nb + 1;
+  if (r.nb > kMaxBatchDims) RFH_THROW(3, "too many batch dimensions");
+  for (int d = 0; d < g.nb; ++d) {
+    r.bdim[d] = g.bdim[d];
+    r.bsi[d] = t.sb[d];
+    r.bso[d] = g.bso[d];
+  }
+  r.bdim[g.nb] = int64_t(n1);
+  r.bsi[g.nb] = t.srow;
+  r.bso[g.nb] = g.eso;
+  r.esi = t.scol;
+  r.eso = g.eso * int64_t(n1);
+  order_dims(r, false);
+  return r;
 }
 
 }  // namespace
@@ -288,14 +433,34 @@ LineSet slice_lines(const LineSet &g, int64_t c0, int64_t cn, size_t scalar) {
   return s;
 }
 
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
 }  // namespace
 
 size_t c2c_tmp_bytes(const C2CPlan &p, const LineSet &g) {
   const size_t elem = p.dbl ? 16 : 8;
   const size_t lines = size_t(chunk_lines(p, g));
-  if (p.kind == kTwoPass) return lines * p.n1 * p.n2 * elem;
-  if (p.kind == kBlueTwoPass) return 2 * lines * p.n1 * p.n2 * elem;
-  return 0;
+  switch (p.kind) {
+    case kTwoPass: return lines * p.n1 * p.n2 * elem;
+    case kBlueTwoPass: return 2 * lines * p.n1 * p.n2 * elem;
+    case kMultiPass: {
+      const size_t own = align256(size_t(g.nlines) * p.n1 * p.n2 * elem);
+      const Tmp4 t = tmp4(g, p.n1, p.n2, nullptr, input_interleaved(g));
+      return own + c2c_tmp_bytes(*p.sub, multi_rows(g, p.n1, t));
+    }
+    case kBlueMulti: {
+      const size_t own = align256(size_t(g.nlines) * p.bn2 * elem);
+      LineSet w{};
+      w.nlines = g.nlines;
+      w.nb = g.nb;
+      for (int d = 0; d < g.nb; ++d) w.bdim[d] = g.bdim[d];
+      const Work wl = work_layout(g, p.bn2, input_interleaved(g));
+      for (int d = 0; d < g.nb; ++d) w.bsi[d] = w.bso[d] = wl.sb[d];
+      w.esi = w.eso = wl.e;
+      return own + c2c_tmp_bytes(*p.sub, w);
+    }
+    default: return 0;
+  }
 }
 
 std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct,
@@ -312,7 +477,10 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
   }
   std::vector<PassLaunch> out;
   const bool dbl = p.dbl;
+  const size_t elem = dbl ? 16 : 8;
   const int sub_conj = dir > 0 ? SUB_CONJ : 0;
+  const int lines = dbl ? 4 : 8;
+  const bool inter = input_interleaved(g);
   if (p.kind == kDirect) {
     PassSpec ps;
     ps.d = &p.main;
@@ -334,89 +502,161 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     ps.nin = int(p.bn2);
     ps.nout = int(p.n);
     out.push_back(PassLaunch{ps, g});
-  } else {
+  } else if (p.kind == kTwoPass) {
+    const uint64_t n1 = p.n1, n2 = p.n2;
+    const Tmp4 t = tmp4(g, n1, n2, tmp, inter);
+    PassSpec a;
+    a.d = &p.pa;
+    a.op = OP_PASSA;
+    a.dir = dir;
+    a.other = int64_t(n2);
+    a.shift = p.shift;
+    a.t0 = p.big_lo.p;
+    a.t1 = p.big_hi.p;
+    a.nin = a.nout = int(n1);
+    a.lines = lines;
+    out.push_back(PassLaunch{a, pass_a(g, n1, n2, g.in, g.esi, g.bsi, t)});
+    PassSpec b;
+    b.d = &p.pb;
+    b.op = OP_PASSB;
+    b.dir = dir;
+    b.fct = fct;
+    b.other = int64_t(n1);
+    b.nin = b.nout = int(n2);
+    b.lines = lines;
+    out.push_back(PassLaunch{b, pass_b(g, n1, n2, t, g.out, g.eso, g.bso)});
+  } else if (p.kind == kMultiPass) {
+    // columns of n1 with the twiddle of N, then the sub-plan of M on the rows
+    const uint64_t n1 = p.n1, M = p.n2;
+    const Tmp4 t = tmp4(g, n1, M, tmp, inter);
+    PassSpec a;
+    a.d = &p.pa;
+    a.op = OP_PASSA;
+    a.dir = dir;
+    a.other = int64_t(M);
+    a.shift = p.shift;
+    a.t0 = p.big_lo.p;
+    a.t1 = p.big_hi.p;
+    a.nin = a.nout = int(n1);
+    a.lines = lines;
+    out.push_back(PassLaunch{a, pass_a(g, n1, M, g.in, g.esi, g.bsi, t)});
+    void *rest = static_cast<char *>(tmp) + align256(size_t(g.nlines) * n1 * M * elem);
+    for (const PassLaunch &pl : describe_c2c(*p.sub, multi_rows(g, n1, t), dir, fct, rest))
+      out.push_back(pl);
+  } else if (p.kind == kBlueTwoPass) {  // Bluestein in four passes through two temporaries
     const uint64_t n1 = p.n1, n2 = p.n2, N = n1 * n2;
-    const int lines = dbl ? 4 : 8;
-    if (p.kind == kTwoPass) {
-      PassSpec a;
-      a.d = &p.pa;
-      a.op = OP_PASSA;
-      a.dir = dir;
-      a.other = int64_t(n2);
-      a.shift = p.shift;
-      a.t0 = p.big_lo.p;
-      a.t1 = p.big_hi.p;
-      a.nin = a.nout = int(n1);
-      a.lines = lines;
-      out.push_back(PassLaunch{a, pass_a_lines(g, n1, n2, g.in, g.esi, tmp)});
-      PassSpec b;
-      b.d = &p.pb;
-      b.op = OP_PASSB;
-      b.dir = dir;
-      b.fct = fct;
-      b.other = int64_t(n1);
-      b.nin = b.nout = int(n2);
-      b.lines = lines;
-      out.push_back(PassLaunch{b, pass_b_lines(g, n1, n2, tmp, g.out, g.eso, g.bso)});
-    } else {  // Bluestein in four passes through two temporaries
-      char *t1 = static_cast<char *>(tmp);
-      char *t2 = t1 + size_t(g.nlines) * N * (dbl ? 16 : 8);
-      LineSet gc{};  // contiguous work lines of length N, g's batch shape
-      gc.nlines = g.nlines;
-      gc.nb = g.nb;
-      for (int d = 0; d < g.nb; ++d) gc.bdim[d] = g.bdim[d];
-      contiguous_like(g, N, gc.bsi);
-      contiguous_like(g, N, gc.bso);
-      gc.esi = gc.eso = 2;
-      PassSpec a;  // x * chirp (zero beyond n), forward columns
-      a.d = &p.pa;
-      a.op = OP_PASSA;
-      a.dir = -1;
-      a.sub = SUB_CHIRP | sub_conj;
-      a.bign = int64_t(p.n);
-      a.other = int64_t(n2);
-      a.shift = p.shift;
-      a.t0 = p.big_lo.p;
-      a.t1 = p.big_hi.p;
-      a.t2 = p.bk.p;
-      a.nin = a.nout = int(n1);
-      a.lines = lines;
-      out.push_back(PassLaunch{a, pass_a_lines(g, n1, n2, g.in, g.esi, t1)});
-      PassSpec b;  // forward rows, times the kernel, natural order into t2
-      b.d = &p.pb;
-      b.op = OP_PASSB;
-      b.dir = -1;
-      b.sub = SUB_KERNEL | sub_conj;
-      b.other = int64_t(n1);
-      b.t1 = p.bkf.p;
-      b.nin = b.nout = int(n2);
-      b.lines = lines;
-      out.push_back(PassLaunch{b, pass_b_lines(gc, n1, n2, t1, t2, 2, gc.bso)});
-      PassSpec a3 = a;  // backward columns
-      a3.dir = +1;
-      a3.sub = 0;
-      a3.t2 = nullptr;
-      out.push_back(PassLaunch{a3, pass_a_lines(gc, n1, n2, t2, 2, t1)});
-      PassSpec b4;  // backward rows, times chirp and fct, first n outputs
-      b4.d = &p.pb;
-      b4.op = OP_PASSB;
-      b4.dir = +1;
-      b4.sub = SUB_FINAL | sub_conj;
-      b4.fct = fct;
-      b4.bign = int64_t(p.n);
-      b4.other = int64_t(n1);
-      b4.t0 = p.bk.p;
-      b4.nin = b4.nout = int(n2);
-      b4.lines = lines;
-      out.push_back(PassLaunch{b4, pass_b_lines(g, n1, n2, t1, g.out, g.eso, g.bso)});
-    }
+    char *t1p = static_cast<char *>(tmp);
+    char *t2p = t1p + size_t(g.nlines) * N * elem;
+    const Tmp4 t = tmp4(g, n1, n2, t1p, inter);
+    const Work w = work_layout(g, N, inter);
+    PassSpec a;  // x * chirp (zero beyond n), forward columns
+    a.d = &p.pa;
+    a.op = OP_PASSA;
+    a.dir = -1;
+    a.sub = SUB_CHIRP | sub_conj;
+    a.bign = int64_t(p.n);
+    a.other = int64_t(n2);
+    a.shift = p.shift;
+    a.t0 = p.big_lo.p;
+    a.t1 = p.big_hi.p;
+    a.t2 = p.bk.p;
+    a.nin = a.nout = int(n1);
+    a.lines = lines;
+    out.push_back(PassLaunch{a, pass_a(g, n1, n2, g.in, g.esi, g.bsi, t)});
+    PassSpec b;  // forward rows, times the kernel, natural order into t2
+    b.d = &p.pb;
+    b.op = OP_PASSB;
+    b.dir = -1;
+    b.sub = SUB_KERNEL | sub_conj;
+    b.other = int64_t(n1);
+    b.t1 = p.bkf.p;
+    b.nin = b.nout = int(n2);
+    b.lines = lines;
+    out.push_back(PassLaunch{b, pass_b(g, n1, n2, t, t2p, w.e, w.sb)});
+    PassSpec a3 = a;  // backward columns
+    a3.dir = +1;
+    a3.sub = 0;
+    a3.t2 = nullptr;
+    out.push_back(PassLaunch{a3, pass_a(g, n1, n2, t2p, w.e, w.sb, t)});
+    PassSpec b4;  // backward rows, times chirp and fct, first n outputs
+    b4.d = &p.pb;
+    b4.op = OP_PASSB;
+    b4.dir = +1;
+    b4.sub = SUB_FINAL | sub_conj;
+    b4.fct = fct;
+    b4.bign = int64_t(p.n);
+    b4.other = int64_t(n1);
+    b4.t0 = p.bk.p;
+    b4.nin = b4.nout = int(n2);
+    b4.lines = lines;
+    out.push_back(PassLaunch{b4, pass_b(g, n1, n2, t, g.out, g.eso, g.bso)});
+  } else {  // kBlueMulti: elementwise chirp and kernel passes around the sub-plan of bn2
+    const uint64_t N = p.bn2;
+    const Work wl = work_layout(g, N, inter);
+    char *wp = static_cast<char *>(tmp);
+    void *rest = wp + align256(size_t(g.nlines) * N * elem);
+    LineSet w = g;  // the work lines, natural order, length bn2
+    w.in = wp;
+    w.out = wp;
+    for (int d = 0; d < g.nb; ++d) w.bsi[d] = w.bso[d] = wl.sb[d];
+    w.esi = w.eso = wl.e;
+    w.auxdim = -1;
+    LineSet gin = g;  // user input -> work lines
+    gin.out = wp;
+    for (int d = 0; d < g.nb; ++d) gin.bso[d] = wl.sb[d];
+    gin.eso = wl.e;
+    LineSet gout = g;  // work lines -> user output
+    gout.in = wp;
+    for (int d = 0; d < g.nb; ++d) gout.bsi[d] = wl.sb[d];
+    gout.esi = wl.e;
+    PassSpec e;
+    e.d = &p.main;
+    e.sub = sub_conj;
+    e.bign = int64_t(p.n);
+    e.t0 = p.bk.p;
+    e.t1 = p.bkf.p;
+    e.op = OP_BLUE_IN;
+    e.other = int64_t(N);
+    out.push_back(PassLaunch{e, gin});
+    for (const PassLaunch &pl : describe_c2c(*p.sub, w, -1, 1.0, rest)) out.push_back(pl);
+    e.op = OP_BLUE_MUL;
+    e.other = int64_t(N);
+    out.push_back(PassLaunch{e, w});
+    for (const PassLaunch &pl : describe_c2c(*p.sub, w, +1, 1.0, rest)) out.push_back(pl);
+    e.op = OP_BLUE_OUT;
+    e.fct = fct;
+    e.other = int64_t(p.n);
+    out.push_back(PassLaunch{e, gout});
   }
   return out;
 }
 
+void check_tmp_spans(const std::vector<PassLaunch> &passes, const void *tmp, size_t bytes,
+                     size_t scalar) {
+  const char *lo = static_cast<const char *>(tmp), *hi = lo + bytes;
+  auto check = [&](const void *ptr, const int64_t *bs, int64_t es, int64_t count,
+                   const LineSet &g) {
+    const char *q = static_cast<const char *>(ptr);
+    if (!tmp || q < lo || q >= hi) return;  // not a workspace access
+    int64_t span = (count - 1) * std::llabs(es);
+    for (int d = 0; d < g.nb; ++d) span += (g.bdim[d] - 1) * std::llabs(bs[d]);
+    if (q + size_t(span + 2) * scalar > hi)
+      RFH_THROW(1, "internal error: a pass would reach past its workspace");
+  };
+  for (const PassLaunch &pl : passes) {
+    const bool elem = pl.ps.op == OP_BLUE_IN || pl.ps.op == OP_BLUE_MUL ||
+                      pl.ps.op == OP_BLUE_OUT;
+    const int64_t cin = elem ? pl.ps.other : pl.ps.nin, cout = elem ? pl.ps.other : pl.ps.nout;
+    check(pl.g.in, pl.g.bsi, pl.g.esi, cin, pl.g);
+    check(pl.g.out, pl.g.bso, pl.g.eso, cout, pl.g);
+  }
+}
+
 void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStream_t s,
                 void *tmp) {
-  for (const PassLaunch &pl : describe_c2c(p, g, dir, fct, tmp)) run_pass_any(p.dbl, pl.ps, pl.g, s);
+  const auto passes = describe_c2c(p, g, dir, fct, tmp);
+  check_tmp_spans(passes, tmp, c2c_tmp_bytes(p, g), p.dbl ? 8 : 4);
+  for (const PassLaunch &pl : passes) run_pass_any(p.dbl, pl.ps, pl.g, s);
 }
 
 void launch_r2r_fused(const C2CPlan &p, const LineSet &g, PreMode pre, PostMode post,

@@ -36,6 +36,10 @@ struct PassLaunch {
 // workspace), in order.
 std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct,
                                      void *tmp);
+// Throws if a pass that reads or writes the workspace [tmp, tmp + bytes)
+// would reach past its end.
+void check_tmp_spans(const std::vector<PassLaunch> &passes, const void *tmp, size_t bytes,
+                     size_t scalar);
 // The single pass of a fused r2c / c2r transform.
 PassLaunch describe_real_fused(const RealPlan &rp, RealOp op, const LineSet &g, bool conj_flag,
                                double fct);

@@ -392,34 +392,99 @@ static void make_two_pass(C2CPlan &p, uint64_t N, size_t budget) {
   upload_any(p.big_hi, hi, p.dbl);
 }
 
-static std::shared_ptr<C2CPlan> build_c2c(uint64_t n, bool dbl, size_t budget) {
+std::shared_ptr<C2CPlan> get_c2c_plan_any(uint64_t n, bool dbl, size_t budget, bool inter);
+
+static bool all_factors_at_most(uint64_t n, uint64_t lim) {
+  return largest_prime_factor(n) <= lim;
+}
+
+// Column length n1 of a multi-pass split N = n1 * M (n1 <= lim, all prime
+// factors of M <= lim): M should take two passes of its own (or more when N
+// exceeds lim^3), so n1 near the cube root of N is preferred.
+static uint64_t pick_multi(uint64_t N, bool dbl, int lines, size_t budget) {
+  const uint64_t lim = direct_max(dbl, lines, budget);
+  const double target = std::cbrt(double(N));
+  uint64_t best = 0;
+  double bestd = 1e300;
+  for (uint64_t d = 2; d <= lim && d <= N / 2; ++d) {
+    if (N % d) continue;
+    const uint64_t M = N / d;
+    if (M <= lim) continue;  // the two-pass split covers it
+    const bool two = pick_split(M, dbl, lines, budget) != 0;
+    if (!two && M <= lim * lim) continue;
+    if (!all_factors_at_most(M, lim)) continue;
+    const double dd = std::fabs(std::log(double(d)) - std::log(target)) + (two ? 0.0 : 100.0);
+    if (dd < bestd) {
+      bestd = dd;
+      best = d;
+    }
+  }
+  return best;
+}
+
+static void make_multi(C2CPlan &p, uint64_t N, uint64_t n1, size_t budget) {
+  p.n1 = n1;
+  p.n2 = N / n1;
+  make_direct(p.pa, n1, p.dbl);
+  int shift = 0;
+  while ((uint64_t(1) << (2 * shift)) < N) ++shift;
+  p.shift = shift;
+  const uint64_t lo_n = uint64_t(1) << shift, hi_n = (N + lo_n - 1) / lo_n;
+  std::vector<std::complex<double>> lo(lo_n), hi(hi_n);
+  for (uint64_t i = 0; i < lo_n; ++i) lo[i] = wfwd(i, N);
+  for (uint64_t i = 0; i < hi_n; ++i) hi[i] = wfwd(i * lo_n, N);
+  upload_any(p.big_lo, lo, p.dbl);
+  upload_any(p.big_hi, hi, p.dbl);
+  // the rows are written with the column index as their fastest output stride
+  p.sub = get_c2c_plan_any(p.n2, p.dbl, budget, true);
+}
+
+static std::shared_ptr<C2CPlan> build_c2c_x(uint64_t n, bool dbl, size_t budget, bool inter) {
   auto p = std::make_shared<C2CPlan>();
   p->n = n;
   p->dbl = dbl;
+  const int lines = two_pass_lines(dbl);
+  // a length fits one LDS pass: any line group for contiguous lines; for
+  // interleaved lines, a group of adjacent lines wide enough for full row
+  // segments (the specialised power-of-two kernel, or `lines` lines)
+  auto direct_ok = [&](uint64_t N) {
+    if (!inter) return N <= direct_max(dbl, 1, budget);
+    return fast_interleaved_ok(N, dbl) || N <= direct_max(dbl, lines, budget);
+  };
   uint64_t bn2 = 0;
   bool blue = n > 1 && plan_bluestein(n, false, &bn2);
-  const uint64_t N = blue ? bn2 : n;
-  if (N <= direct_max(dbl, 1, budget)) {
-    p->kind = blue ? kBlueDirect : kDirect;
-    make_direct(p->main, N, dbl);
-  } else if (pick_split(N, dbl, two_pass_lines(dbl), budget)) {
-    p->kind = blue ? kBlueTwoPass : kTwoPass;
-    make_two_pass(*p, N, budget);
-  } else if (!blue) {
-    // a long length the two-pass split cannot factor: Bluestein always can
+  uint64_t n1 = 0;
+  if (!blue) {
+    if (direct_ok(n)) {
+      p->kind = kDirect;
+      make_direct(p->main, n, dbl);
+      return p;
+    }
+    if (pick_split(n, dbl, lines, budget)) {
+      p->kind = kTwoPass;
+      make_two_pass(*p, n, budget);
+      return p;
+    }
+    if ((n1 = pick_multi(n, dbl, lines, budget)) != 0) {
+      p->kind = kMultiPass;
+      make_multi(*p, n, n1, budget);
+      return p;
+    }
+    // a long length with a prime factor too large for any LDS pass
     blue = true;
     bn2 = good_size_cmplx(2 * n - 1);
-    if (bn2 <= direct_max(dbl, 1, budget)) {
-      p->kind = kBlueDirect;
-      make_direct(p->main, bn2, dbl);
-    } else {
-      p->kind = kBlueTwoPass;
-      make_two_pass(*p, bn2, budget);
-    }
-  } else {
-    RFH_THROW(3, "unsupported transform length " + std::to_string(n));
   }
-  if (blue) make_bluestein(*p, n, bn2);
+  if (direct_ok(bn2)) {
+    p->kind = kBlueDirect;
+    make_direct(p->main, bn2, dbl);
+  } else if (pick_split(bn2, dbl, lines, budget)) {
+    p->kind = kBlueTwoPass;
+    make_two_pass(*p, bn2, budget);
+  } else {
+    p->kind = kBlueMulti;
+    p->sub = get_c2c_plan_any(bn2, dbl, budget, false);
+  }
+  make_bluestein(*p, n, bn2);
   return p;
 }
 
@@ -454,21 +519,23 @@ static std::shared_ptr<RealPlan> build_real(uint64_t n, bool dbl, size_t budget)
 
 namespace {
 std::mutex g_cache_mu;
-std::map<std::tuple<int, uint64_t, bool, size_t>, std::shared_ptr<C2CPlan>> g_c2c;
-std::map<std::tuple<int, uint64_t, bool, size_t>, std::shared_ptr<RealPlan>> g_real;
+typedef std::tuple<int, uint64_t, bool, size_t, bool> PlanKey;
+std::map<PlanKey, std::shared_ptr<C2CPlan>> g_c2c;
+std::map<PlanKey, std::shared_ptr<RealPlan>> g_real;
 
 template <typename P, typename B>
-std::shared_ptr<P> cached(std::map<std::tuple<int, uint64_t, bool, size_t>, std::shared_ptr<P>> &m,
-                          uint64_t n, bool dbl, size_t budget, B &&build) {
+std::shared_ptr<P> cached(std::map<PlanKey, std::shared_ptr<P>> &m, uint64_t n, bool dbl,
+                          size_t budget, bool inter, B &&build) {
   int dev = 0;
   HIP_CHECK(hipGetDevice(&dev));
-  auto key = std::make_tuple(dev, n, dbl, budget);
+  const size_t b = budget ? budget : lds_dynamic_budget();
+  const PlanKey key = std::make_tuple(dev, n, dbl, b, inter);
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);
     auto it = m.find(key);
     if (it != m.end()) return it->second;
   }
-  auto p = build(n, dbl, budget ? budget : lds_dynamic_budget());  // outside the lock
+  auto p = build(n, dbl, b, inter);  // outside the lock (sub-plans recurse)
   std::lock_guard<std::mutex> lk(g_cache_mu);
   auto it = m.find(key);
   if (it != m.end()) return it->second;
@@ -477,12 +544,17 @@ std::shared_ptr<P> cached(std::map<std::tuple<int, uint64_t, bool, size_t>, std:
 }
 }  // namespace
 
-std::shared_ptr<C2CPlan> get_c2c_plan(uint64_t n, bool dbl, size_t budget) {
-  return cached(g_c2c, n, dbl, budget, build_c2c);
+std::shared_ptr<C2CPlan> get_c2c_plan_any(uint64_t n, bool dbl, size_t budget, bool inter) {
+  return cached(g_c2c, n, dbl, budget, inter, build_c2c_x);
+}
+
+std::shared_ptr<C2CPlan> get_c2c_plan(uint64_t n, bool dbl, size_t budget, bool interleaved) {
+  return get_c2c_plan_any(n, dbl, budget, interleaved);
 }
 
 std::shared_ptr<RealPlan> get_real_plan(uint64_t n, bool dbl, size_t budget) {
-  return cached(g_real, n, dbl, budget, build_real);
+  return cached(g_real, n, dbl, budget, false,
+                [](uint64_t n_, bool d_, size_t b_, bool) { return build_real(n_, d_, b_); });
 }
 
 }  // namespace rfh

@@ -197,6 +197,8 @@ std::unique_ptr<Built> build(const std::vector<rfh_job_t> &jobs, bool dbl, size_
     const int dir = j.forward ? -1 : +1;
     if (j.kind == RFH_JOB_C2C) {
       per[i] = describe_c2c(*cplan[i], one_line(j.in, j.out, 2, 2), dir, j.fct, ar + tmp_off[i]);
+      check_tmp_spans(per[i], ar + tmp_off[i],
+                      c2c_tmp_bytes(*cplan[i], one_line(nullptr, nullptr, 2, 2)), dbl ? 8 : 4);
       continue;
     }
     const bool r2c = j.kind == RFH_JOB_R2C;
@@ -218,9 +220,11 @@ std::unique_ptr<Built> build(const std::vector<rfh_job_t> &jobs, bool dbl, size_
                               : (half ? kPostRealPairs : kPostReal);
     pre.sub = pre_mode | (!r2c && j.forward ? R2R_CONJ : 0);
     per[i].push_back(PassLaunch{pre, one_line(j.in, work, r2c ? 1 : 2, 2)});
-    for (auto &pl : describe_c2c(*cplan[i], one_line(work, work, 2, 2), r2c ? -1 : +1, 1.0,
-                                 ar + tmp_off[i]))
-      per[i].push_back(pl);
+    const auto cp = describe_c2c(*cplan[i], one_line(work, work, 2, 2), r2c ? -1 : +1, 1.0,
+                                 ar + tmp_off[i]);
+    check_tmp_spans(cp, ar + tmp_off[i],
+                    c2c_tmp_bytes(*cplan[i], one_line(nullptr, nullptr, 2, 2)), dbl ? 8 : 4);
+    for (const auto &pl : cp) per[i].push_back(pl);
     PassSpec post{};
     post.d = &cplan[i]->main;
     post.op = OP_POST;
@@ -239,6 +243,10 @@ std::unique_ptr<Built> build(const std::vector<rfh_job_t> &jobs, bool dbl, size_
     jd[i].first = int(pd.size());
     jd[i].npass = int(per[i].size());
     for (const PassLaunch &pl : per[i]) {
+      if (pl.ps.op == OP_BLUE_IN || pl.ps.op == OP_BLUE_MUL || pl.ps.op == OP_BLUE_OUT)
+        RFH_THROW(3, "ragged batch: length " + std::to_string(jobs[i].n) +
+                         " needs the multi-pass Bluestein form, which the persistent kernel "
+                         "does not run");
       LineArgs<T> a = make_line_args<T>(pl.ps, pl.g);
       if (pl.ps.op == OP_PRE || pl.ps.op == OP_POST) {
         a.f.n = int(cplan[i]->n);  // work-line length m

@@ -77,7 +77,15 @@ struct Direct {
   DevBuf gen;  // roots for radices 7, 11, 13 and generic odd primes
 };
 
-enum PlanKind { kDirect = 0, kBlueDirect = 1, kTwoPass = 2, kBlueTwoPass = 3 };
+enum PlanKind {
+  kDirect = 0,      // one LDS-resident FFT per line
+  kBlueDirect = 1,  // Bluestein, its length-bn2 FFTs in LDS
+  kTwoPass = 2,     // four-step N = n1 * n2, both passes in LDS
+  kBlueTwoPass = 3, // Bluestein in four passes (two four-step FFTs of bn2)
+  kMultiPass = 4,   // N = n1 * M: one column pass of n1, then the plan `sub` of M on the rows
+  kBlueMulti = 5,   // Bluestein whose length bn2 needs `sub` (multi-pass): elementwise
+                    // chirp / kernel passes around two transforms of bn2
+};
 
 // Complex transform plan for one length and precision on one device.
 struct C2CPlan {
@@ -91,6 +99,7 @@ struct C2CPlan {
   int shift = 0;
   uint64_t bn2 = 0;      // Bluestein length (0 = none)
   DevBuf bk, bkf;        // chirp e^{+i pi m^2/n} (n) and DFT(chirp)/bn2 (bn2)
+  std::shared_ptr<const C2CPlan> sub;  // kMultiPass: rows of M = n2; kBlueMulti: length bn2
 };
 
 // Real-input plan used by the fused r2c / c2r kernels: even n runs a
@@ -108,10 +117,17 @@ struct RealPlan {
 
 // process-wide switch of the specialised power-of-two kernels (fast.hip)
 extern bool g_fast_enabled;
+// whether the specialised kernel has a form for n (power of two) with
+// interleaved lines
+bool fast_interleaved_ok(uint64_t n, bool dbl);
 
 // Plans whose LDS passes fit `lds_budget` bytes of dynamic LDS (0 = the
-// device limit minus the kernels' static LDS).
-std::shared_ptr<C2CPlan> get_c2c_plan(uint64_t n, bool dbl, size_t lds_budget = 0);
+// device limit minus the kernels' static LDS).  `interleaved`: the lines
+// lie side by side in memory (a strided axis), so a line group must hold
+// enough adjacent lines for wide row segments; lengths whose direct form
+// cannot then run as four-step passes instead.
+std::shared_ptr<C2CPlan> get_c2c_plan(uint64_t n, bool dbl, size_t lds_budget = 0,
+                                      bool interleaved = false);
 std::shared_ptr<RealPlan> get_real_plan(uint64_t n, bool dbl, size_t lds_budget = 0);
 
 // Radix schedule (execution order) for an LDS-resident FFT of length n.
@@ -141,6 +157,7 @@ struct LineSet {
   int64_t bsi[kMaxBatchDims] = {};
   int64_t bso[kMaxBatchDims] = {};
   int64_t esi = 0, eso = 0;  // element strides (scalar units)
+  int auxdim = -1;           // batch dim whose index the four-step ops use (-1: nb - 1)
 };
 
 // Kernel launchers (kernels.hip).  All enqueue on `s`.

@@ -270,3 +270,42 @@ def test_two_pass_chunked(rows, n, dtype):
     torch.cuda.synchronize()
     e = float(torch.linalg.norm(y.to(torch.complex128) - ref) / torch.linalg.norm(ref))
     assert e <= 3 * tol_of(dtype), e
+
+
+@pytest.mark.parametrize("shape,dtype", [((8192, 40), np.complex64), ((5000, 33), np.complex64),
+                                         ((3000, 17), np.complex128), ((4099, 20), np.complex64),
+                                         ((4099, 12), np.complex128), ((2048, 24), np.complex128)])
+def test_strided_long_axis(shape, dtype):
+    """Long lines along a strided axis (lines interleaved in memory): lengths
+    whose direct form cannot keep enough adjacent lines in one LDS group run
+    as four-step passes with a line-fastest intermediate (4099: Bluestein)."""
+    rs = np.random.RandomState(shape[0] % 1000)
+    x = crandn(rs, shape, dtype)
+    tol = 3 * tol_of(dtype)
+    check("c2c", x, x, np.array([0]), True, 1.0, tol=tol)
+    check("c2c", x, x, np.array([0]), False, 1.0 / shape[0], tol=tol)
+
+
+@pytest.mark.parametrize("n,rows,dtype,kind", [
+    (1 << 21, 2, np.complex64, 4), (3 << 20, 1, np.complex128, 4), (1 << 22, 1, np.complex64, 4),
+    (1000003, 1, np.complex64, 5), (1000003, 1, np.complex128, 5)])
+def test_multi_pass_lengths(n, rows, dtype, kind):
+    """Lengths beyond one two-pass split: a column pass and the rows by
+    their own plan (kind 4), and primes whose Bluestein length needs it
+    (kind 5); against numpy's FP64 transform, both directions, and along a
+    strided axis."""
+    k, _, n1, n2 = R.gpu_schedule(n, dtype == np.complex128)
+    print(f"case multi-pass n={n} {dtype.__name__} schedule={k, n1, n2}", flush=True)
+    assert k == kind
+    rs = np.random.RandomState(n % 997)
+    x = crandn(rs, (rows, n), dtype)
+    tol = 5 * tol_of(dtype)
+    want = np.fft.fft(x.astype(np.complex128), axis=1)
+    got = on_gpu(R, "c2c", x, x, np.array([1]), True, 1.0)
+    assert rel_l2(got, want) <= tol
+    got = on_gpu(R, "c2c", x, x, np.array([1]), False, 1.0 / n)
+    assert rel_l2(got, np.fft.ifft(x.astype(np.complex128), axis=1)) <= tol
+    if rows == 2:
+        xt = np.ascontiguousarray(x.T)
+        got = on_gpu(R, "c2c", xt, xt, np.array([0]), True, 1.0)
+        assert rel_l2(got, want.T) <= tol
