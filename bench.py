@@ -40,6 +40,13 @@ BYTES_R2C = N3 ** 3 * 4 + N3 * N3 * (N3 // 2 + 1) * 8
 BYTES_DCT = 2 * N2D * N2D * 8
 BYTES_STEP = 2 * BYTES_R2C + BYTES_DCT
 BYTES_C2C_PASS = 2 * N3 * N3 * (N3 // 2 + 1) * 8
+# HBM bytes per launch of the dominant kernel, measured with
+#   rocprofv3 --pmc FETCH_SIZE -- python3 bench.py --probe   (then WRITE_SIZE)
+# each counter scaled by what it reads for the device copy of known size
+# (67,633,152 B each way) in the same run: FETCH_SIZE x 1.999, WRITE_SIZE
+# x 1.000 on MI355X.  Mean of six launches: 67,779,649 B fetched +
+# 69,065,136 B written = 1.012 x the algorithmic bytes.
+TRAFFIC_C2C_PASS = 136844785
 
 
 def rel_l2(a, b):
@@ -274,6 +281,9 @@ def main():
     ap.add_argument("--probe", action="store_true",
                     help="only launch the dominant kernel (3x per strided axis) after a "
                          "3x device copy of the same 135 MB array, for counter runs")
+    ap.add_argument("--probe-two-pass", action="store_true",
+                    help="only run the complex64 64 x 2^20 transform (axis 1) twice, for "
+                         "kernel-trace and counter runs of the two-pass form")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: every transform of this workload fits one "
@@ -286,6 +296,14 @@ def main():
 
     if args.probe:
         probe(torch, R)
+        return
+    if args.probe_two_pass:
+        a = torch.randn((64, 1 << 20), dtype=torch.complex64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(2):
+            R.c2c(a, b, np.array([1]), True, 1.0, 1)
+        torch.cuda.synchronize()
+        print("probe done", flush=True)
         return
     x_h, d_h = inputs()
     dev = torch.device("cuda:0")
@@ -375,11 +393,14 @@ def main():
         "ms_per_step_eager": round(eager_ms, 4),
         "parity": {"roundtrip_rel_l2": rt, "dct_vs_oracle_rel_l2": dct_err,
                    "r2c_slab_vs_numpy_fp64_rel_l2": slab_err},
-        "roofline": {"bound": "hbm", "kernel": "k_lines<float, OP_C2C> (strided c2c, n=256)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "k_fast<float, 256, 16 threads/line, 16 lines/block, interleaved> "
+                               "(c2c over strided axis 0 / 1 of the 256x256x129 half spectrum)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "avg_launch_ms": round(kms, 5),
-                     "bytes_per_launch": BYTES_C2C_PASS, "traffic": None},
+                     "bytes_per_launch": BYTES_C2C_PASS, "traffic": TRAFFIC_C2C_PASS,
+                     "traffic_over_algorithmic": round(TRAFFIC_C2C_PASS / BYTES_C2C_PASS, 3)},
     }
     if not args.no_extra:
         res["configs"] = extra_configs(torch, R, stream, 10)
