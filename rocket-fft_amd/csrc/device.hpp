@@ -87,6 +87,34 @@ __device__ __forceinline__ void dft3(C2<T> *v, T dir) {
   v[2] = a - b;
 }
 
+// Radix 9 as 3 x 3: DFT3 over x[n2], x[n2+3], x[n2+6] for each n2, twiddle
+// W9^(n2 k1), DFT3 over n2; X[k1 + 3 k2].  W9^m = cos(2 pi m/9) + dir i sin(2 pi m/9).
+template <typename T>
+__device__ __forceinline__ void dft9(C2<T> *v, T dir) {
+  const T c1 = T(0.766044443118978035202392650555416674),
+          s1 = T(0.642787609686539326322643409907263433),
+          c2 = T(0.173648177666930348851716626769314796),
+          s2 = T(0.984807753012208059366743024589523014),
+          c4 = T(-0.93969262078590838405410927732473147),
+          s4 = T(0.342020143325668733044099614682259581);
+  C2<T> a[3] = {v[0], v[3], v[6]}, b[3] = {v[1], v[4], v[7]}, c[3] = {v[2], v[5], v[8]};
+  dft3(a, dir);
+  dft3(b, dir);
+  dft3(c, dir);
+  b[1] = cmul(b[1], C2<T>{c1, dir * s1});
+  b[2] = cmul(b[2], C2<T>{c2, dir * s2});
+  c[1] = cmul(c[1], C2<T>{c2, dir * s2});
+  c[2] = cmul(c[2], C2<T>{c4, dir * s4});
+#pragma unroll
+  for (int k1 = 0; k1 < 3; ++k1) {
+    C2<T> t[3] = {a[k1], b[k1], c[k1]};
+    dft3(t, dir);
+    v[k1] = t[0];
+    v[k1 + 3] = t[1];
+    v[k1 + 6] = t[2];
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void dft4(C2<T> *v, T dir) {
   const C2<T> t0 = v[0] + v[2], t1 = v[0] - v[2], t2 = v[1] + v[3];
@@ -169,8 +197,7 @@ __device__ __forceinline__ void dftp(C2<T> *v, T dir, const C2<T> *__restrict__ 
     d[m - 1] = v[m] - v[P - m];
     sum = sum + t[m - 1];
   }
-  C2<T> out[P];
-  out[0] = sum;
+  v[0] = sum;  // the inputs live on in x0, t and d
 #pragma unroll
   for (int k = 1; k <= H; ++k) {
     C2<T> a = x0, b = {T(0), T(0)};
@@ -181,11 +208,9 @@ __device__ __forceinline__ void dftp(C2<T> *v, T dir, const C2<T> *__restrict__ 
       b = b + w.y * d[m - 1];
     }
     const C2<T> rb = rot(b, dir);
-    out[k] = a + rb;
-    out[P - k] = a - rb;
+    v[k] = a + rb;
+    v[P - k] = a - rb;
   }
-#pragma unroll
-  for (int k = 0; k < P; ++k) v[k] = out[k];
 }
 
 template <int R, typename T>
@@ -195,6 +220,7 @@ __device__ __forceinline__ void dft(C2<T> *v, T dir, const C2<T> *g) {
   else if constexpr (R == 4) dft4(v, dir);
   else if constexpr (R == 5) dft5(v, dir);
   else if constexpr (R == 8) dft8(v, dir);
+  else if constexpr (R == 9) dft9(v, dir);
   else if constexpr (R == 16) dft16(v, dir);
   else dftp<R>(v, dir, g);
 }
@@ -311,8 +337,10 @@ __device__ C2<T> *run_fft(const FftDesc &f, C2<T> *a, C2<T> *b, int L, int ld,
       case 5: stage_fixed<5>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
       case 7: stage_fixed<7>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
       case 8: stage_fixed<8>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 9: stage_fixed<9>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
       case 11: stage_fixed<11>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
       case 13: stage_fixed<13>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 17: stage_fixed<17>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
       case 16: stage_fixed<16>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
       default: stage_generic(a, b, L, ld, f.n, st.radix, st.ns, t, g, dir); break;
     }

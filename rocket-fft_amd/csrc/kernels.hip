@@ -58,6 +58,9 @@ int choose_lines(int ld, size_t elem, int64_t nlines, bool interleaved) {
   while (L * 2 <= kMaxLinesPerGroup && size_t(2) * (L * 2) * ld * elem <= budget &&
          int64_t(L) < nlines)
     L *= 2;
+  // contiguous lines: fewer lines per group when that leaves CUs idle
+  if (!interleaved)
+    while (L > 1 && nlines < int64_t(L) * 512) L /= 2;
   return L;
 }
 
@@ -435,6 +438,19 @@ LineSet slice_lines(const LineSet &g, int64_t c0, int64_t cn, size_t scalar) {
 
 size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 
+// Lines per group of a four-step pass over lines of n: at least `base`
+// adjacent lines (64-byte row segments), more for short lines until a group
+// holds 2048 elements or its two LDS buffers reach 64 KB (or the plan's budget).
+int pass_lines(const C2CPlan &p, uint64_t n, int base) {
+  const size_t elem = p.dbl ? 16 : 8;
+  const size_t cap = std::min<size_t>(p.budget ? p.budget : 65536, 65536);
+  int L = base;
+  while (L * 2 <= kMaxLinesPerGroup && uint64_t(L) * n < 2048 &&
+         size_t(2) * size_t(L * 2) * n * elem <= cap)
+    L *= 2;
+  return L;
+}
+
 }  // namespace
 
 size_t c2c_tmp_bytes(const C2CPlan &p, const LineSet &g) {
@@ -514,7 +530,7 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     a.t0 = p.big_lo.p;
     a.t1 = p.big_hi.p;
     a.nin = a.nout = int(n1);
-    a.lines = lines;
+    a.lines = pass_lines(p, n1, lines);
     out.push_back(PassLaunch{a, pass_a(g, n1, n2, g.in, g.esi, g.bsi, t)});
     PassSpec b;
     b.d = &p.pb;
@@ -523,7 +539,7 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     b.fct = fct;
     b.other = int64_t(n1);
     b.nin = b.nout = int(n2);
-    b.lines = lines;
+    b.lines = pass_lines(p, n2, lines);
     out.push_back(PassLaunch{b, pass_b(g, n1, n2, t, g.out, g.eso, g.bso)});
   } else if (p.kind == kMultiPass) {
     // columns of n1 with the twiddle of N, then the sub-plan of M on the rows
@@ -538,7 +554,7 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     a.t0 = p.big_lo.p;
     a.t1 = p.big_hi.p;
     a.nin = a.nout = int(n1);
-    a.lines = lines;
+    a.lines = pass_lines(p, n1, lines);
     out.push_back(PassLaunch{a, pass_a(g, n1, M, g.in, g.esi, g.bsi, t)});
     void *rest = static_cast<char *>(tmp) + align256(size_t(g.nlines) * n1 * M * elem);
     for (const PassLaunch &pl : describe_c2c(*p.sub, multi_rows(g, n1, t), dir, fct, rest))
@@ -561,7 +577,7 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     a.t1 = p.big_hi.p;
     a.t2 = p.bk.p;
     a.nin = a.nout = int(n1);
-    a.lines = lines;
+    a.lines = pass_lines(p, n1, lines);
     out.push_back(PassLaunch{a, pass_a(g, n1, n2, g.in, g.esi, g.bsi, t)});
     PassSpec b;  // forward rows, times the kernel, natural order into t2
     b.d = &p.pb;
@@ -571,7 +587,7 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     b.other = int64_t(n1);
     b.t1 = p.bkf.p;
     b.nin = b.nout = int(n2);
-    b.lines = lines;
+    b.lines = pass_lines(p, n2, lines);
     out.push_back(PassLaunch{b, pass_b(g, n1, n2, t, t2p, w.e, w.sb)});
     PassSpec a3 = a;  // backward columns
     a3.dir = +1;
@@ -588,7 +604,7 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     b4.other = int64_t(n1);
     b4.t0 = p.bk.p;
     b4.nin = b4.nout = int(n2);
-    b4.lines = lines;
+    b4.lines = pass_lines(p, n2, lines);
     out.push_back(PassLaunch{b4, pass_b(g, n1, n2, t, g.out, g.eso, g.bso)});
   } else {  // kBlueMulti: elementwise chirp and kernel passes around the sub-plan of bn2
     const uint64_t N = p.bn2;

@@ -6,13 +6,14 @@
 //   :2629-2652) and the FFTPACK-vs-Bluestein decision (:2843-2861,
 //   :2885-2903).  The GPU follows the reference's Bluestein decision and its
 //   Bluestein length n2 = good_size_cmplx(2n-1) (:2765-2767).
-// * The GPU's own radix schedule (radix 16 first, then 8/4/2, then odd
-//   primes), independent of the reference's FFTPACK order: both compute the
-//   same DFT, the schedule only changes rounding.
+// * The GPU's own radix schedule (radix 16 first, then 8/4/2, then 9 for
+//   each pair of threes, then odd primes), independent of the reference's
+//   FFTPACK order: both compute the same DFT, the schedule only changes
+//   rounding.
 // * Twiddle, chirp and Bluestein-kernel tables generated in FP64 on the host
 //   (rounded to FP32 for complex64 plans) and kept resident in HBM for the
 //   life of the process, in a thread-safe cache keyed by (device, length,
-//   precision).  There is no LRU cap: the tables are O(n) and HBM is large.
+//   precision, LDS budget, interleaved lines).  There is no LRU cap: the tables are O(n) and HBM is large.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -199,7 +200,14 @@ std::vector<int> gpu_radices(uint64_t n) {
   if (twos == 3) r.push_back(8);
   if (twos == 2) r.push_back(4);
   if (twos == 1) r.push_back(2);
-  for (uint64_t d = 3; d * d <= n; d += 2)
+  int threes = 0;
+  while (n % 3 == 0) {
+    n /= 3;
+    ++threes;
+  }
+  for (; threes >= 2; threes -= 2) r.push_back(9);
+  if (threes) r.push_back(3);
+  for (uint64_t d = 5; d * d <= n; d += 2)
     while (n % d == 0) {
       r.push_back(int(d));
       n /= d;
@@ -273,7 +281,7 @@ static void upload_any(DevBuf &b, const std::vector<std::complex<double>> &v, bo
 }
 
 static bool has_fixed_kernel(int r) {
-  return r == 2 || r == 3 || r == 4 || r == 5 || r == 8 || r == 16;
+  return r == 2 || r == 3 || r == 4 || r == 5 || r == 8 || r == 9 || r == 16;
 }
 
 static void make_direct(Direct &d, uint64_t n, bool dbl) {
@@ -443,6 +451,7 @@ static std::shared_ptr<C2CPlan> build_c2c_x(uint64_t n, bool dbl, size_t budget,
   auto p = std::make_shared<C2CPlan>();
   p->n = n;
   p->dbl = dbl;
+  p->budget = budget;
   const int lines = two_pass_lines(dbl);
   // a length fits one LDS pass: any line group for contiguous lines; for
   // interleaved lines, a group of adjacent lines wide enough for full row

@@ -100,6 +100,7 @@ struct C2CPlan {
   uint64_t bn2 = 0;      // Bluestein length (0 = none)
   DevBuf bk, bkf;        // chirp e^{+i pi m^2/n} (n) and DFT(chirp)/bn2 (bn2)
   std::shared_ptr<const C2CPlan> sub;  // kMultiPass: rows of M = n2; kBlueMulti: length bn2
+  size_t budget = 0;     // dynamic LDS the plan's passes may use per workgroup
 };
 
 // Real-input plan used by the fused r2c / c2r kernels: even n runs a
