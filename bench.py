@@ -406,7 +406,7 @@ def main():
         res["configs"] = extra_configs(torch, R, stream, 10)
         res["configs"]["ragged_4096_c64_r2c"] = ragged_config(torch, R, stream)
         res["configs"]["precision_c64_16384x4096"] = precision_config(torch, R, stream)
-        res["configs"]["lengths_batch256"] = lengths_config(torch, R, stream, 10)
+        res["configs"]["lengths_batch256"] = lengths_config(torch, R, stream, 50)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(x_h, d_h, args.cpu_seconds)
     print(json.dumps(res))

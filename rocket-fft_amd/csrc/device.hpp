@@ -368,13 +368,17 @@ enum LineOp {
   OP_BLUE_IN = 11,   // work[m] = x[m] * conj(chirp[m]) for m < n, zero up to bn2
   OP_BLUE_MUL = 12,  // work[m] *= kernel[m]
   OP_BLUE_OUT = 13,  // y[m] = fct * work[m] * conj(chirp[m]) for m < n
+  // Bluestein rows of the three-pass form: forward FFT of the row, times the
+  // kernel t2 at m = aux + k * other, inverse FFT, times the inverse four-step
+  // twiddle (tables t0 / t1); written back in place
+  OP_BLUEROW = 14,
 };
 // OP_R2R packs its modes into LineArgs::sub
 constexpr int R2R_ORTHO = 1 << 16, R2R_CONJ = 1 << 17, R2R_FLIP = 1 << 18;
 enum SubFlags {
   SUB_CHIRP = 1,   // pass A: multiply loads by the chirp, zero beyond n
   SUB_KERNEL = 2,  // pass B: multiply by the Bluestein kernel (natural order)
-  SUB_FINAL = 4,   // pass B: multiply by the chirp and fct, keep m < n
+  SUB_FINAL = 4,   // pass A or B: multiply by the chirp and fct, keep m = aux + k other < n
   SUB_CONJ = 8,    // Bluestein backward / r2c conj output / c2r conj input
 };
 
@@ -649,10 +653,12 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
   C2<T> *bufB = bufA + a.L * a.ld;
   const int64_t b0 = tile * a.L;
   const int nl = int(min(int64_t(a.L), a.nlines - b0));
-  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+  // offsets of the group's lines; the slots of missing lines (the last
+  // group of a line set) hold zeros and are never used to address memory
+  for (int l = threadIdx.x; l < a.L; l += blockDim.x) {
     int64_t rem = b0 + l, oi = 0, oo = 0;
     int aux = 0;
-    for (int d = a.nb - 1; d >= 0; --d) {
+    for (int d = l < nl ? a.nb - 1 : -1; d >= 0; --d) {
       const int64_t q = rem / a.bdim[d];
       const int64_t idx = rem - q * a.bdim[d];
       rem = q;
@@ -672,7 +678,7 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
   for_tile(nl, a.nin, a.lfast_in, a.log2L, [&](int l, int j) {
     const T *base = a.in + s_oi[l];
     C2<T> v;
-    if constexpr (OP == OP_C2C || OP == OP_PASSB) {
+    if constexpr (OP == OP_C2C || OP == OP_PASSB || OP == OP_BLUEROW) {
       v = ldc<VIN>(base + j * a.esi);
     } else if constexpr (OP == OP_PASSA) {
       if (a.sub & SUB_CHIRP) {
@@ -735,10 +741,10 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
 
   // ----------------------------------------------------------------- FFT
   C2<T> *res = run_fft<T>(a.f, bufA, bufB, a.L, ld, a.tw, a.gen, a.dir);
-  if constexpr (OP == OP_BLUE) {
+  if constexpr (OP == OP_BLUE || OP == OP_BLUEROW) {
     const int n2 = a.f.n;
-    for_tile(a.L, n2, 0, a.log2L, [&](int l, int k) {
-      const C2<T> w = a.t1[k];
+    for_tile(nl, n2, 0, a.log2L, [&](int l, int k) {
+      const C2<T> w = OP == OP_BLUE ? a.t1[k] : a.t2[s_aux[l] + int64_t(k) * a.other];
       res[l * ld + k] = cmul(res[l * ld + k], conjf ? conjc(w) : w);
     });
     __syncthreads();
@@ -752,7 +758,18 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
     if constexpr (OP == OP_C2C) {
       stc<VOUT>(base + k * a.eso, a.fct * r[k]);
     } else if constexpr (OP == OP_PASSA) {
-      const C2<T> w = dirw(big_twiddle(a, int64_t(k) * s_aux[l]), a.dir);
+      if (a.sub & SUB_FINAL) {  // Bluestein columns, last pass: chirp and fct, m < n
+        const int64_t m = s_aux[l] + int64_t(k) * a.other;
+        if (m < a.bign) {
+          const C2<T> c = a.t0[m];
+          stc<VOUT>(base + k * a.eso, a.fct * cmul(r[k], conjf ? c : conjc(c)));
+        }
+      } else {
+        const C2<T> w = dirw(big_twiddle(a, int64_t(k) * s_aux[l]), a.dir);
+        stc<VOUT>(base + k * a.eso, cmul(r[k], w));
+      }
+    } else if constexpr (OP == OP_BLUEROW) {
+      const C2<T> w = dirw(big_twiddle(a, int64_t(k) * s_aux[l]), T(1));
       stc<VOUT>(base + k * a.eso, cmul(r[k], w));
     } else if constexpr (OP == OP_PASSB) {
       const int64_t m = s_aux[l] + int64_t(k) * a.other;

@@ -107,8 +107,16 @@ __device__ __forceinline__ void fstore(const LineArgs<T> &a, T *gout, int aux, i
   if constexpr (OP == OP_C2C) {
     stc<VOUT>(gout + p * a.eso, a.fct * v);
   } else if constexpr (OP == OP_PASSA) {
-    const C2<T> w = dirw(big_twiddle(a, int64_t(p) * aux), a.dir);
-    stc<VOUT>(gout + p * a.eso, cmul(v, w));
+    if (a.sub & SUB_FINAL) {  // Bluestein columns, last pass: chirp and fct, m < n
+      const int64_t m = aux + int64_t(p) * a.other;
+      if (m < a.bign) {
+        const C2<T> ch = a.t0[m];
+        stc<VOUT>(gout + p * a.eso, a.fct * cmul(v, (a.sub & SUB_CONJ) ? ch : conjc(ch)));
+      }
+    } else {
+      const C2<T> w = dirw(big_twiddle(a, int64_t(p) * aux), a.dir);
+      stc<VOUT>(gout + p * a.eso, cmul(v, w));
+    }
   } else if constexpr (OP == OP_PASSB) {
     const int64_t m = aux + int64_t(p) * a.other;
     const bool cj = (a.sub & SUB_CONJ) != 0;

@@ -241,6 +241,7 @@ void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
     case OP_PASSA: launch_var<T, OP_PASSA>(a, lds, vin, vout, s); break;
     case OP_PASSB: launch_var<T, OP_PASSB>(a, lds, vin, vout, s); break;
     case OP_BLUE: launch_var<T, OP_BLUE>(a, lds, vin, vout, s); break;
+    case OP_BLUEROW: launch_var<T, OP_BLUEROW>(a, lds, vin, vout, s); break;
     case OP_R2C_EVEN: launch_var<T, OP_R2C_EVEN>(a, lds, vin, vout, s); break;
     case OP_R2C_ODD: launch_var<T, OP_R2C_ODD>(a, lds, false, vout, s); break;
     case OP_C2R_EVEN: launch_var<T, OP_C2R_EVEN>(a, lds, vin, vout, s); break;
@@ -387,6 +388,54 @@ LineSet pass_b(const LineSet &g, uint64_t n1, uint64_t n2, const Tmp4 &t, void *
   return b;
 }
 
+// Rows of the intermediate t transformed in place (lines (b, k1), n2
+// elements).  aux = k1.
+LineSet rows_inplace(const LineSet &g, uint64_t n1, uint64_t n2, const Tmp4 &t) {
+  LineSet r{};
+  r.in = t.p;
+  r.out = t.p;
+  r.nlines = g.nlines * int64_t(n1);
+  r.nb = g.nb + 1;
+  if (r.nb > kMaxBatchDims) RFH_THROW(3, "too many batch dimensions");
+  for (int d = 0; d < g.nb; ++d) {
+    r.bdim[d] = g.bdim[d];
+    r.bsi[d] = r.bso[d] = t.sb[d];
+  }
+  r.bdim[g.nb] = int64_t(n1);
+  r.bsi[g.nb] = r.bso[g.nb] = t.srow;
+  r.esi = r.eso = t.scol;
+  r.auxdim = g.nb;
+  order_dims(r, false);
+  (void)n2;
+  return r;
+}
+
+// Columns of the intermediate t (lines (b, c), n1 elements) written to
+// position c + n2 * r of the output line.  aux = c.
+LineSet pass_c(const LineSet &g, uint64_t n1, uint64_t n2, const Tmp4 &t, void *out,
+               int64_t eso_line, const int64_t *outbs) {
+  LineSet c{};
+  c.in = t.p;
+  c.out = out;
+  c.nlines = g.nlines * int64_t(n2);
+  c.nb = g.nb + 1;
+  if (c.nb > kMaxBatchDims) RFH_THROW(3, "too many batch dimensions");
+  for (int d = 0; d < g.nb; ++d) {
+    c.bdim[d] = g.bdim[d];
+    c.bsi[d] = t.sb[d];
+    c.bso[d] = outbs[d];
+  }
+  c.bdim[g.nb] = int64_t(n2);
+  c.bsi[g.nb] = t.scol;
+  c.bso[g.nb] = eso_line;
+  c.esi = t.srow;
+  c.eso = eso_line * int64_t(n2);
+  c.auxdim = g.nb;
+  order_dims(c, true);
+  (void)n1;
+  return c;
+}
+
 // Rows of a multi-pass intermediate t (lines (b, k1) of length M) as the
 // input of the sub-plan, output to position k1 + n1 * k' of g's lines.
 LineSet multi_rows(const LineSet &g, uint64_t n1, const Tmp4 &t) {
@@ -422,7 +471,7 @@ constexpr size_t kChunkBytes = size_t(64) << 20;
 
 int64_t chunk_lines(const C2CPlan &p, const LineSet &g) {
   if (p.kind != kTwoPass && p.kind != kBlueTwoPass) return g.nlines;
-  const size_t per = size_t(p.n1) * p.n2 * (p.dbl ? 16 : 8) * (p.kind == kBlueTwoPass ? 2 : 1);
+  const size_t per = size_t(p.n1) * p.n2 * (p.dbl ? 16 : 8);
   const int64_t c = std::max<int64_t>(1, int64_t(kChunkBytes / per));
   return (g.nb == 1 && g.nlines > c) ? c : g.nlines;
 }
@@ -458,7 +507,7 @@ size_t c2c_tmp_bytes(const C2CPlan &p, const LineSet &g) {
   const size_t lines = size_t(chunk_lines(p, g));
   switch (p.kind) {
     case kTwoPass: return lines * p.n1 * p.n2 * elem;
-    case kBlueTwoPass: return 2 * lines * p.n1 * p.n2 * elem;
+    case kBlueTwoPass: return lines * p.n1 * p.n2 * elem;
     case kMultiPass: {
       const size_t own = align256(size_t(g.nlines) * p.n1 * p.n2 * elem);
       const Tmp4 t = tmp4(g, p.n1, p.n2, nullptr, input_interleaved(g));
@@ -559,13 +608,13 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     void *rest = static_cast<char *>(tmp) + align256(size_t(g.nlines) * n1 * M * elem);
     for (const PassLaunch &pl : describe_c2c(*p.sub, multi_rows(g, n1, t), dir, fct, rest))
       out.push_back(pl);
-  } else if (p.kind == kBlueTwoPass) {  // Bluestein in four passes through two temporaries
-    const uint64_t n1 = p.n1, n2 = p.n2, N = n1 * n2;
-    char *t1p = static_cast<char *>(tmp);
-    char *t2p = t1p + size_t(g.nlines) * N * elem;
-    const Tmp4 t = tmp4(g, n1, n2, t1p, inter);
-    const Work w = work_layout(g, N, inter);
-    PassSpec a;  // x * chirp (zero beyond n), forward columns
+  } else if (p.kind == kBlueTwoPass) {
+    // Bluestein in three passes over one intermediate t (n1 x n2 per line):
+    // columns (x * chirp, forward, twiddle); rows in place (forward, kernel,
+    // inverse, inverse twiddle); columns (inverse, chirp and fct, m < n).
+    const uint64_t n1 = p.n1, n2 = p.n2;
+    const Tmp4 t = tmp4(g, n1, n2, tmp, inter);
+    PassSpec a;
     a.d = &p.pa;
     a.op = OP_PASSA;
     a.dir = -1;
@@ -579,33 +628,31 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     a.nin = a.nout = int(n1);
     a.lines = pass_lines(p, n1, lines);
     out.push_back(PassLaunch{a, pass_a(g, n1, n2, g.in, g.esi, g.bsi, t)});
-    PassSpec b;  // forward rows, times the kernel, natural order into t2
-    b.d = &p.pb;
-    b.op = OP_PASSB;
-    b.dir = -1;
-    b.sub = SUB_KERNEL | sub_conj;
-    b.other = int64_t(n1);
-    b.t1 = p.bkf.p;
-    b.nin = b.nout = int(n2);
-    b.lines = pass_lines(p, n2, lines);
-    out.push_back(PassLaunch{b, pass_b(g, n1, n2, t, t2p, w.e, w.sb)});
-    PassSpec a3 = a;  // backward columns
-    a3.dir = +1;
-    a3.sub = 0;
-    a3.t2 = nullptr;
-    out.push_back(PassLaunch{a3, pass_a(g, n1, n2, t2p, w.e, w.sb, t)});
-    PassSpec b4;  // backward rows, times chirp and fct, first n outputs
-    b4.d = &p.pb;
-    b4.op = OP_PASSB;
-    b4.dir = +1;
-    b4.sub = SUB_FINAL | sub_conj;
-    b4.fct = fct;
-    b4.bign = int64_t(p.n);
-    b4.other = int64_t(n1);
-    b4.t0 = p.bk.p;
-    b4.nin = b4.nout = int(n2);
-    b4.lines = pass_lines(p, n2, lines);
-    out.push_back(PassLaunch{b4, pass_b(g, n1, n2, t, g.out, g.eso, g.bso)});
+    PassSpec r;
+    r.d = &p.pb;
+    r.op = OP_BLUEROW;
+    r.dir = -1;
+    r.sub = sub_conj;
+    r.other = int64_t(n1);
+    r.shift = p.shift;
+    r.t0 = p.big_lo.p;
+    r.t1 = p.big_hi.p;
+    r.t2 = p.bkf.p;
+    r.nin = r.nout = int(n2);
+    r.lines = pass_lines(p, n2, lines);
+    out.push_back(PassLaunch{r, rows_inplace(g, n1, n2, t)});
+    PassSpec c;
+    c.d = &p.pa;
+    c.op = OP_PASSA;
+    c.dir = +1;
+    c.sub = SUB_FINAL | sub_conj;
+    c.fct = fct;
+    c.bign = int64_t(p.n);
+    c.other = int64_t(n2);
+    c.t0 = p.bk.p;
+    c.nin = c.nout = int(n1);
+    c.lines = pass_lines(p, n1, lines);
+    out.push_back(PassLaunch{c, pass_c(g, n1, n2, t, g.out, g.eso, g.bso)});
   } else {  // kBlueMulti: elementwise chirp and kernel passes around the sub-plan of bn2
     const uint64_t N = p.bn2;
     const Work wl = work_layout(g, N, inter);

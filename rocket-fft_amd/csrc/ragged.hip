@@ -96,6 +96,7 @@ __global__ void __launch_bounds__(kThreads) k_ragged(const JobDev *__restrict__ 
           case OP_PASSA: line_body<T, OP_PASSA, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
           case OP_PASSB: line_body<T, OP_PASSB, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
           case OP_BLUE: line_body<T, OP_BLUE, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+          case OP_BLUEROW: line_body<T, OP_BLUEROW, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
           case OP_R2C_EVEN: line_body<T, OP_R2C_EVEN, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
           case OP_R2C_ODD: line_body<T, OP_R2C_ODD, false, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
           case OP_C2R_EVEN: line_body<T, OP_C2R_EVEN, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
