@@ -543,6 +543,32 @@ __device__ __forceinline__ C2<T> pre_elem(int mode, const T *x, int64_t es, int6
     case kPreRealPairs:
       if (2 * j < n) v = C2<T>{x[2 * j * es], x[(2 * j + 1) * es]};
       break;
+    case kPreDct2N:
+    case kPreDst2N: {
+      // v_j = x_2j for 2j < n, v_{n-1-j} = x_2j+1: V = FFT_n(v) gives
+      // DCT-II_k = 2 Re(e^{-i pi k/2n} V_k); DST-II is DCT-II of (-1)^j x_j
+      // read backwards
+      const int64_t i = 2 * j < n ? 2 * j : 2 * (n - 1 - j) + 1;
+      T xv = x[i * es];
+      if (mode == kPreDst2N && (i & 1)) xv = -xv;
+      v.x = xv;
+      break;
+    }
+    case kPreDct3N:
+    case kPreDst3N: {
+      // a = x (DCT-III) or x reversed (DST-III), a_n = 0, ortho scales x_0 by
+      // sqrt 2; V_j = (a_j - i a_{n-j}) e^{i pi j/2n} and the inverse FFT of
+      // length n holds the even outputs up and the odd outputs down
+      auto at = [&](int64_t q) -> T {
+        if (q >= n) return T(0);
+        const int64_t src = mode == kPreDct3N ? q : n - 1 - q;
+        T xv = x[src * es];
+        if (ortho && src == 0) xv *= sqrt2;
+        return xv;
+      };
+      v = cmul(C2<T>{at(j), -at(n - j)}, expipi<T>(j, 2 * n));
+      break;
+    }
     case kPreHalfSpectrum: {
       // z_j = Σ_k [A_k + i e^{2πik/n} D_k] e^{2πikj/(n/2)} with A = X_k + conj X_{h-k},
       // D = X_k - conj X_{h-k}: the inverse FFT of length h = n/2 gives
@@ -636,6 +662,19 @@ __device__ __forceinline__ T post_elem(int mode, G &&get, int64_t n, int64_t k, 
       break;
     case kPostRealPairs:
       r = (k % 2) ? get(k / 2).y : get(k / 2).x;
+      break;
+    case kPostDct2N:
+    case kPostDst2N: {
+      const int64_t q = mode == kPostDct2N ? k : n - 1 - k;
+      const C2<T> z = cmul(get(q), expipi<T>(-q, 2 * n));
+      r = T(2) * z.x;
+      if (ortho && k == 0) r *= isqrt2;
+      break;
+    }
+    case kPostDct3N:
+    case kPostDst3N:
+      r = get((k % 2 == 0) ? k / 2 : n - 1 - (k - 1) / 2).x;
+      if (mode == kPostDst3N && (k & 1)) r = -r;
       break;
   }
   return r;

@@ -539,13 +539,15 @@ void dcst(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout, arystruct_t 
           post = cosine ? kPostDct1 : kPostDst1;
           m = cosine ? 2 * (n - 1) : 2 * (n + 1);
           break;
-        case 2:
-          pre = kPreType2;
-          post = cosine ? kPostDct2 : kPostDst2;
+        case 2:  // an n-point FFT of the reordered input
+          pre = cosine ? kPreDct2N : kPreDst2N;
+          post = cosine ? kPostDct2N : kPostDst2N;
+          m = n;
           break;
-        case 3:
-          pre = cosine ? kPreDct3 : kPreDst3;
-          post = cosine ? kPostDct3 : kPostDst3;
+        case 3:  // an n-point inverse FFT, outputs reordered
+          pre = cosine ? kPreDct3N : kPreDst3N;
+          post = cosine ? kPostDct3N : kPostDst3N;
+          m = n;
           fwd = false;
           break;
         default:

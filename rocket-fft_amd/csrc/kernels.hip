@@ -58,9 +58,10 @@ int choose_lines(int ld, size_t elem, int64_t nlines, bool interleaved) {
   while (L * 2 <= kMaxLinesPerGroup && size_t(2) * (L * 2) * ld * elem <= budget &&
          int64_t(L) < nlines)
     L *= 2;
-  // contiguous lines: fewer lines per group when that leaves CUs idle
-  if (!interleaved)
-    while (L > 1 && nlines < int64_t(L) * 512) L /= 2;
+  // fewer lines per group when the groups would leave CUs idle (down to one
+  // line for contiguous lines, two adjacent lines for interleaved ones)
+  const int floor_l = interleaved ? 2 : 1;
+  while (L > floor_l && nlines < int64_t(L) * 512) L /= 2;
   return L;
 }
 

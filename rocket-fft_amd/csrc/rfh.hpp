@@ -188,6 +188,10 @@ enum PreMode {
   kPreHalfcomplex,        // FFTPACK halfcomplex -> Hermitian spectrum
   kPreRealPairs,          // r2c, even n: a_j = x_2j + i x_2j+1 (j < n/2)
   kPreHalfSpectrum,       // c2r, even n: work line of the length-n/2 inverse FFT
+  kPreDct2N,              // DCT-II through an n-point FFT: even inputs up, odd inputs down
+  kPreDst2N,              // DST-II: the same with the odd inputs negated
+  kPreDct3N,              // DCT-III through an n-point inverse FFT: (x_j - i x_{n-j}) e^{i pi j/2n}
+  kPreDst3N,              // DST-III: the same over the reversed input
 };
 enum PostMode {
   kPostComplexHalf = 0,   // bins 0..n/2 (r2c)
@@ -204,6 +208,10 @@ enum PostMode {
   kPostPackHalfcomplex,   // bins -> FFTPACK halfcomplex
   kPostHalfSpectrum,      // r2c, even n: bins 0..n/2 from the length-n/2 spectrum
   kPostRealPairs,         // c2r, even n: y_2j = Re z_j, y_2j+1 = Im z_j
+  kPostDct2N,             // DCT-II: 2 Re(e^{-i pi k/2n} V_k)
+  kPostDst2N,             // DST-II: the same at n-1-k
+  kPostDct3N,             // DCT-III: y_2m = Re v_m, y_2m+1 = Re v_{n-1-m}
+  kPostDst3N,             // DST-III: the same with odd outputs negated
 };
 struct RealPrePost {
   int64_t n = 0;      // logical line length (real)

@@ -138,6 +138,10 @@ def test_dct2_ortho_2d(dtype):
     x = rs.uniform(-1, 1, (64, 48)).astype(dtype)
     check("dct", x, x, np.array([0, 1]), 2, 1.0 / np.sqrt(4 * 64 * 48), True)
     check("dst", x, x, np.array([1, 0]), 3, 1.0, True, inplace=True)
+    x = rs.uniform(-1, 1, (512, 512)).astype(dtype)  # the bench image
+    check("dct", x, x, np.array([0, 1]), 2, 1.0 / 1024, True)
+    check("dct", x, x, np.array([1, 0]), 3, 1.0 / 1024, True)
+    check("dst", x, x, np.array([0, 1]), 2, 1.0, True)
 
 
 @pytest.mark.parametrize("r2h", [False, True])
@@ -237,7 +241,8 @@ def test_two_pass_pow2_fast(dtype):
 
 
 @pytest.mark.parametrize("kind,type,n", [("dct", 2, 4099), ("dst", 3, 4099), ("dct", 4, 8192),
-                                         ("dct", 1, 6000), ("dst", 1, 5000)])
+                                         ("dct", 1, 6000), ("dst", 1, 5000), ("dct", 3, 12000),
+                                         ("dst", 2, 12000)])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_dct_long_unfused(kind, type, n, dtype):
     """Work lengths that need Bluestein or two passes run pre -> c2c -> post."""
