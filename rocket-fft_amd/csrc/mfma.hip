@@ -2,28 +2,30 @@
 // complex matrix products on the gfx950 matrix cores.
 //
 // A radix-16 butterfly is X = F v with F[k][r] = e^{dir 2 pi i k r / 16}.
-// Sixteen butterflies of a wave are one matrix product D (16x16) = A (16x32)
-// x B (32x16) with the complex product embedded in real arithmetic:
-//   B = [Re v; Im v] (one column per butterfly),
-//   A_re = [Re F, -Im F]  ->  D_re = Re X,     A_im = [Im F, Re F]  ->  D_im = Im X.
+// Sixteen butterflies of a wave are one complex matrix product D = F B
+// (B: one column per butterfly), done as four real products
+//   D_re = F_re B_re + (-F_im) B_im,      D_im = F_im B_re + F_re B_im
+// with v_mfma_f32_16x16x16_{f16,bf16} and FP32 accumulation.  Each lane
+// holds four inputs of one butterfly, both parts, so every value is loaded
+// and twiddled once.
 // FP16 form: every operand is split x = hi + lo (two FP16 values, ~22 bits),
-// and D accumulates hi*hi + hi*lo + lo*hi in FP32 (v_mfma_f32_16x16x32_f16);
-// each butterfly is first scaled by a power of two so its largest input is
-// just below 2^10 (exact, undone after the product), keeping FP16 in range.
+// and D accumulates hi*hi + hi*lo + lo*hi; each butterfly is first scaled by
+// a power of two so its largest input is just below 2^10 (exact, undone
+// after the product), keeping FP16 in range.
 // BF16 form: bf16 has FP32's range but 8 mantissa bits, so operands are
 // split three ways (hi + mid + lo) and the six products down to 2^-24 are
-// accumulated (v_mfma_f32_16x16x32_bf16); a two-way BF16 split (three
-// products, ~16 bits) is kept as a precision-study mode.
+// accumulated; a two-way BF16 split (three products, ~16 bits) is kept as a
+// precision-study mode.
 //
-// Operand layout of v_mfma_f32_16x16x32_{f16,bf16} (64 lanes; lane l, g =
-// l / 16): A row l % 16, columns 8g..8g+7; B rows 8g..8g+7, column l % 16;
+// Operand layout of v_mfma_f32_16x16x16_{f16,bf16} (64 lanes; lane l, g =
+// l / 16): A row l % 16, columns 4g..4g+3; B rows 4g..4g+3, column l % 16;
 // D rows 4g..4g+3, column l % 16.  k_mfma_probe lets a GPU test check this
 // layout against a host matrix product.
 //
 // The kernel covers complex64 lines of 4096 (three radix-16 stages, 256
-// threads per line, one line per workgroup); everything else about it
-// (global loads of stage 0, LDS exchanges with the padded layout, twiddle
-// tables, stores of the last stage) is the specialised kernel of fast.hip.
+// threads per line, one line per workgroup); the global loads of stage 0,
+// LDS exchanges with the padded layout, twiddle tables and stores of the
+// last stage follow the specialised kernel of fast.hip.
 #include "passes.hpp"
 
 namespace rfh {
@@ -33,31 +35,31 @@ int g_precision_mode = 0;  // 0 plain FP32, 1 FP16 split, 2 BF16 3-split, 3 BF16
 namespace {
 using namespace dev;
 
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf168 __attribute__((ext_vector_type(8)));
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf164 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 enum { MODE_F16 = 1, MODE_BF16_3 = 2, MODE_BF16_2 = 3 };
 
 __global__ void k_mfma_probe(int bf, const float *a, const float *b, float *d) {
-  // a, b: 64 lanes x 8 values in the operand layout above; d: 64 x 4
+  // a, b: 64 lanes x 4 values in the operand layout above; d: 64 x 4
   const int l = threadIdx.x;
   f4 c = {0.f, 0.f, 0.f, 0.f};
   f4 r;
   if (bf) {
-    bf168 x, y;
-    for (int i = 0; i < 8; ++i) {
-      x[i] = (__bf16)a[l * 8 + i];
-      y[i] = (__bf16)b[l * 8 + i];
+    bf164 x, y;
+    for (int i = 0; i < 4; ++i) {
+      x[i] = (__bf16)a[l * 4 + i];
+      y[i] = (__bf16)b[l * 4 + i];
     }
-    r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, y, c, 0, 0, 0);
+    r = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(x, y, c, 0, 0, 0);
   } else {
-    half8 x, y;
-    for (int i = 0; i < 8; ++i) {
-      x[i] = (_Float16)a[l * 8 + i];
-      y[i] = (_Float16)b[l * 8 + i];
+    half4 x, y;
+    for (int i = 0; i < 4; ++i) {
+      x[i] = (_Float16)a[l * 4 + i];
+      y[i] = (_Float16)b[l * 4 + i];
     }
-    r = __builtin_amdgcn_mfma_f32_16x16x32_f16(x, y, c, 0, 0, 0);
+    r = __builtin_amdgcn_mfma_f32_16x16x16f16(x, y, c, 0, 0, 0);
   }
   for (int i = 0; i < 4; ++i) d[l * 4 + i] = r[i];
 }
@@ -67,30 +69,30 @@ struct Ops;
 
 template <>
 struct Ops<MODE_F16> {
-  typedef half8 V;
+  typedef half4 V;
   static constexpr int terms = 2;
   __device__ static V split(const float *x, int t) {
     V v;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < 4; ++i) {
       const _Float16 h = (_Float16)x[i];
       v[i] = t == 0 ? h : (_Float16)(x[i] - (float)h);
     }
     return v;
   }
   __device__ static f4 mma(V a, V b, f4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
   }
 };
 
 template <int TERMS>
 struct OpsBf {
-  typedef bf168 V;
+  typedef bf164 V;
   static constexpr int terms = TERMS;
   __device__ static V split(const float *x, int t) {
     V v;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < 4; ++i) {
       float r = x[i];
       __bf16 p = (__bf16)r;
       for (int s = 0; s < t; ++s) {
@@ -102,7 +104,7 @@ struct OpsBf {
     return v;
   }
   __device__ static f4 mma(V a, V b, f4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
   }
 };
 template <>
@@ -121,37 +123,38 @@ __device__ __forceinline__ float group_max(float m) {
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(256) k_fft4096_mfma(const LineArgs<float> a) {
+__global__ void __launch_bounds__(256, 4) k_fft4096_mfma(const LineArgs<float> a) {
   using O = Ops<MODE>;
   typedef typename O::V V;
   constexpr int N = 4096, NR = 256;
   __shared__ C2<float> line[N + (N >> 4) + 1];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col = lane & 15, kg = lane >> 4;
-  const int part = kg >> 1, rbase = (kg & 1) * 8;
   const float dir = a.dir;
   const int64_t b = xcd_tile();
   const bool valid = b < a.nlines;
   const float *gin = a.in + b * a.bsi[a.nb > 0 ? a.nb - 1 : 0];
   float *gout = a.out + b * a.bso[a.nb > 0 ? a.nb - 1 : 0];
 
-  // A operands: rows = output k (lane % 16), columns 8*kg..8*kg+7 of [Re|Im]
-  V Are[O::terms], Aim[O::terms];
+  // A operands: row = output k (lane % 16), columns r = 4 kg .. 4 kg + 3:
+  // F_re, F_im and -F_im, each split into terms
+  V Fr[O::terms], Fi[O::terms], Fn[O::terms];
   {
-    float re[8], im[8];
+    float re[4], im[4], ne[4];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int kk = 8 * kg + i, r = kk & 15;
-      float s, c;
-      sincospif(float((col * r) & 15) / 8.0f, &s, &c);  // angle 2 pi k r / 16
-      const float fr = c, fi = dir * s;
-      re[i] = kk < 16 ? fr : -fi;
-      im[i] = kk < 16 ? fi : fr;
+    for (int i = 0; i < 4; ++i) {
+      const int r = 4 * kg + i;
+      float sn, cs;
+      sincospif(float((col * r) & 15) / 8.0f, &sn, &cs);  // angle 2 pi k r / 16
+      re[i] = cs;
+      im[i] = dir * sn;
+      ne[i] = -im[i];
     }
 #pragma unroll
     for (int t = 0; t < O::terms; ++t) {
-      Are[t] = O::split(re, t);
-      Aim[t] = O::split(im, t);
+      Fr[t] = O::split(re, t);
+      Fi[t] = O::split(im, t);
+      Fn[t] = O::split(ne, t);
     }
   }
 
@@ -161,23 +164,34 @@ __global__ void __launch_bounds__(256) k_fft4096_mfma(const LineArgs<float> a) {
     const C2<float> *tw = a.tw + a.f.st[s].twoff;
     f4 dre[4], dim[4];
     float inv[4];
+    // all of this lane's inputs and twiddles of the stage first (loads in
+    // flight together)
+    C2<float> in[4][4], w[4][4];
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
       const int j = wave * 64 + g * 16 + col;
       const int k = j % NS;
-      float x[8];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = 4 * kg + i;
+        if (s == 0)
+          in[g][i] = valid ? ldc<true>(gin + 2 * (j + r * NR)) : C2<float>{0.f, 0.f};
+        else
+          in[g][i] = line[padf(j + r * NR)];
+        w[g][i] = (s > 0 && r > 0) ? tw[(r - 1) * NS + k] : C2<float>{1.f, 0.f};
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      float xr[4], xi[4];
       float m = 0.f;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int r = rbase + i;
-        C2<float> v;
-        if (s == 0)
-          v = valid ? ldc<true>(gin + 2 * (j + r * NR)) : C2<float>{0.f, 0.f};
-        else
-          v = line[padf(j + r * NR)];
-        if (s > 0 && r > 0) v = cmul(v, dirw(tw[(r - 1) * NS + k], dir));
-        x[i] = part ? v.y : v.x;
-        m = fmaxf(m, fabsf(x[i]));
+      for (int i = 0; i < 4; ++i) {
+        C2<float> v = in[g][i];
+        if (s > 0) v = cmul(v, dirw(w[g][i], dir));
+        xr[i] = v.x;
+        xi[i] = v.y;
+        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
       }
       m = group_max(m);
       int ex = 0;
@@ -185,10 +199,16 @@ __global__ void __launch_bounds__(256) k_fft4096_mfma(const LineArgs<float> a) {
       const float sc = m > 0.f ? ldexpf(1.f, 10 - ex) : 1.f;
       inv[g] = 1.f / sc;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) x[i] *= sc;
-      V B[O::terms];
+      for (int i = 0; i < 4; ++i) {
+        xr[i] *= sc;
+        xi[i] *= sc;
+      }
+      V Br[O::terms], Bi[O::terms];
 #pragma unroll
-      for (int t = 0; t < O::terms; ++t) B[t] = O::split(x, t);
+      for (int t = 0; t < O::terms; ++t) {
+        Br[t] = O::split(xr, t);
+        Bi[t] = O::split(xi, t);
+      }
       f4 zr = {0.f, 0.f, 0.f, 0.f}, zi = {0.f, 0.f, 0.f, 0.f};
       // products down to the split's precision: sum of term indices < terms
 #pragma unroll
@@ -196,8 +216,10 @@ __global__ void __launch_bounds__(256) k_fft4096_mfma(const LineArgs<float> a) {
 #pragma unroll
         for (int tb = 0; tb < O::terms; ++tb)
           if (ta + tb < O::terms) {
-            zr = O::mma(Are[ta], B[tb], zr);
-            zi = O::mma(Aim[ta], B[tb], zi);
+            zr = O::mma(Fr[ta], Br[tb], zr);
+            zr = O::mma(Fn[ta], Bi[tb], zr);
+            zi = O::mma(Fi[ta], Br[tb], zi);
+            zi = O::mma(Fr[ta], Bi[tb], zi);
           }
       dre[g] = zr;
       dim[g] = zi;
