@@ -391,7 +391,7 @@ LineSet pass_b(const LineSet &g, uint64_t n1, uint64_t n2, const Tmp4 &t, void *
 
 // Rows of the intermediate t transformed in place (lines (b, k1), n2
 // elements).  aux = k1.
-LineSet rows_inplace(const LineSet &g, uint64_t n1, uint64_t n2, const Tmp4 &t) {
+LineSet rows_inplace(const LineSet &g, uint64_t n1, const Tmp4 &t) {
   LineSet r{};
   r.in = t.p;
   r.out = t.p;
@@ -407,14 +407,13 @@ LineSet rows_inplace(const LineSet &g, uint64_t n1, uint64_t n2, const Tmp4 &t) 
   r.esi = r.eso = t.scol;
   r.auxdim = g.nb;
   order_dims(r, false);
-  (void)n2;
   return r;
 }
 
 // Columns of the intermediate t (lines (b, c), n1 elements) written to
 // position c + n2 * r of the output line.  aux = c.
-LineSet pass_c(const LineSet &g, uint64_t n1, uint64_t n2, const Tmp4 &t, void *out,
-               int64_t eso_line, const int64_t *outbs) {
+LineSet pass_c(const LineSet &g, uint64_t n2, const Tmp4 &t, void *out, int64_t eso_line,
+               const int64_t *outbs) {
   LineSet c{};
   c.in = t.p;
   c.out = out;
@@ -433,7 +432,6 @@ LineSet pass_c(const LineSet &g, uint64_t n1, uint64_t n2, const Tmp4 &t, void *
   c.eso = eso_line * int64_t(n2);
   c.auxdim = g.nb;
   order_dims(c, true);
-  (void)n1;
   return c;
 }
 
@@ -641,7 +639,7 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     r.t2 = p.bkf.p;
     r.nin = r.nout = int(n2);
     r.lines = pass_lines(p, n2, lines);
-    out.push_back(PassLaunch{r, rows_inplace(g, n1, n2, t)});
+    out.push_back(PassLaunch{r, rows_inplace(g, n1, t)});
     PassSpec c;
     c.d = &p.pa;
     c.op = OP_PASSA;
@@ -653,7 +651,7 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     c.t0 = p.bk.p;
     c.nin = c.nout = int(n1);
     c.lines = pass_lines(p, n1, lines);
-    out.push_back(PassLaunch{c, pass_c(g, n1, n2, t, g.out, g.eso, g.bso)});
+    out.push_back(PassLaunch{c, pass_c(g, n2, t, g.out, g.eso, g.bso)});
   } else {  // kBlueMulti: elementwise chirp and kernel passes around the sub-plan of bn2
     const uint64_t N = p.bn2;
     const Work wl = work_layout(g, N, inter);

@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(256, 4) k_fft4096_mfma(const LineArgs<float> a
       int ex = 0;
       (void)frexpf(m, &ex);
       const float sc = m > 0.f ? ldexpf(1.f, 10 - ex) : 1.f;
-      inv[g] = 1.f / sc;
+      inv[g] = m > 0.f ? ldexpf(1.f, ex - 10) : 1.f;  // exact: sc is a power of two
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         xr[i] *= sc;
