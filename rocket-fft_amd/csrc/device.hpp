@@ -322,36 +322,6 @@ __device__ void stage_generic(C2<T> *__restrict__ src, C2<T> *__restrict__ dst, 
   }
 }
 
-// Runs all stages; returns the buffer holding the natural-order result.
-template <typename T>
-__device__ C2<T> *run_fft(const FftDesc &f, C2<T> *a, C2<T> *b, int L, int ld,
-                          const C2<T> *__restrict__ tw, const C2<T> *__restrict__ gen, T dir) {
-  for (int s = 0; s < f.nst; ++s) {
-    const Stage st = f.st[s];
-    const C2<T> *t = tw + st.twoff;
-    const C2<T> *g = gen + st.genoff;
-    switch (st.radix) {
-      case 2: stage_fixed<2>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
-      case 3: stage_fixed<3>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
-      case 4: stage_fixed<4>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
-      case 5: stage_fixed<5>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
-      case 7: stage_fixed<7>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
-      case 8: stage_fixed<8>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
-      case 9: stage_fixed<9>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
-      case 11: stage_fixed<11>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
-      case 13: stage_fixed<13>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
-      case 17: stage_fixed<17>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
-      case 16: stage_fixed<16>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
-      default: stage_generic(a, b, L, ld, f.n, st.radix, st.ns, t, g, dir); break;
-    }
-    __syncthreads();
-    C2<T> *x = a;
-    a = b;
-    b = x;
-  }
-  return a;
-}
-
 // ----------------------------------------------------------- line kernel
 
 enum LineOp {
@@ -680,10 +650,152 @@ __device__ __forceinline__ T post_elem(int mode, G &&get, int64_t n, int64_t k, 
   return r;
 }
 
+// Stage 0 of a line group fed straight from global memory: butterfly u of
+// the group takes inputs j + r NR of line l through `load` (the op's
+// pre-processing), and writes its radix-R outputs to dst (ns = 1: no
+// twiddles, output position j R + r).  Missing lines (l >= nl) get zeros.
+template <int R, typename T, typename Ld>
+__device__ __forceinline__ void stage0_in(Ld &&load, C2<T> *__restrict__ dst, int L, int nl,
+                                          int ld, int N, int lfast, int log2L,
+                                          const C2<T> *__restrict__ g, T dir) {
+  const int NR = N / R;
+  const int total = L * NR;
+  const float invNR = 1.0f / float(NR);
+  for (int u = threadIdx.x; u < total; u += blockDim.x) {
+    int l, j;
+    if (lfast) {
+      l = u & ((1 << log2L) - 1);
+      j = u >> log2L;
+    } else {
+      l = fdiv(u, NR, invNR);
+      j = u - l * NR;
+    }
+    C2<T> v[R];
+    if (l < nl) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = load(l, j + r * NR);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = C2<T>{T(0), T(0)};
+    }
+    dft<R>(v, dir, g);
+    C2<T> *d = dst + l * ld + j * R;
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[r] = v[r];
+  }
+}
+
+// Last stage of a line group written straight to global memory: butterfly
+// j (< ns, NR = ns) of line l reads src[j + r ns], twiddles by W^{j r},
+// and hands natural-order bin j + r ns to `store` (the op's post-processing).
+template <int R, typename T, typename St>
+__device__ __forceinline__ void stage_last_out(St &&store, const C2<T> *__restrict__ src,
+                                               int L, int nl, int ld, int ns, int lfast,
+                                               int log2L, const C2<T> *__restrict__ tw,
+                                               const C2<T> *__restrict__ g, T dir) {
+  const int total = L * ns;
+  const float invns = 1.0f / float(ns);
+  for (int u = threadIdx.x; u < total; u += blockDim.x) {
+    int l, j;
+    if (lfast) {
+      l = u & ((1 << log2L) - 1);
+      j = u >> log2L;
+    } else {
+      l = fdiv(u, ns, invns);
+      j = u - l * ns;
+    }
+    if (l >= nl) continue;
+    const C2<T> *s = src + l * ld + j;
+    C2<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = s[r * ns];
+    if (ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], dirw(tw[(r - 1) * ns + j], dir));
+    }
+    dft<R>(v, dir, g);
+#pragma unroll
+    for (int r = 0; r < R; ++r) store(l, j + r * ns, v[r]);
+  }
+}
+
+// Radices whose first / last stage may be fused with global memory: the
+// hand-written butterflies (the symmetric-pair ones would raise the register
+// count of every line kernel)
+__device__ __forceinline__ bool fusable_radix(int r) {
+  return r == 2 || r == 3 || r == 4 || r == 5 || r == 8 || r == 9 || r == 16;
+}
+
+template <typename T, typename Ld>
+__device__ __forceinline__ void stage0_in_any(int R, Ld &&load, C2<T> *dst, int L, int nl,
+                                              int ld, int N, int lfast, int log2L,
+                                              const C2<T> *g, T dir) {
+  switch (R) {
+    case 2: stage0_in<2>(load, dst, L, nl, ld, N, lfast, log2L, g, dir); break;
+    case 3: stage0_in<3>(load, dst, L, nl, ld, N, lfast, log2L, g, dir); break;
+    case 4: stage0_in<4>(load, dst, L, nl, ld, N, lfast, log2L, g, dir); break;
+    case 5: stage0_in<5>(load, dst, L, nl, ld, N, lfast, log2L, g, dir); break;
+    case 8: stage0_in<8>(load, dst, L, nl, ld, N, lfast, log2L, g, dir); break;
+    case 9: stage0_in<9>(load, dst, L, nl, ld, N, lfast, log2L, g, dir); break;
+    default: stage0_in<16>(load, dst, L, nl, ld, N, lfast, log2L, g, dir); break;
+  }
+}
+
+template <typename T, typename St>
+__device__ __forceinline__ void stage_last_out_any(int R, St &&store, const C2<T> *src, int L,
+                                                   int nl, int ld, int ns, int lfast,
+                                                   int log2L, const C2<T> *tw,
+                                                   const C2<T> *g, T dir) {
+  switch (R) {
+    case 2: stage_last_out<2>(store, src, L, nl, ld, ns, lfast, log2L, tw, g, dir); break;
+    case 3: stage_last_out<3>(store, src, L, nl, ld, ns, lfast, log2L, tw, g, dir); break;
+    case 4: stage_last_out<4>(store, src, L, nl, ld, ns, lfast, log2L, tw, g, dir); break;
+    case 5: stage_last_out<5>(store, src, L, nl, ld, ns, lfast, log2L, tw, g, dir); break;
+    case 8: stage_last_out<8>(store, src, L, nl, ld, ns, lfast, log2L, tw, g, dir); break;
+    case 9: stage_last_out<9>(store, src, L, nl, ld, ns, lfast, log2L, tw, g, dir); break;
+    default: stage_last_out<16>(store, src, L, nl, ld, ns, lfast, log2L, tw, g, dir); break;
+  }
+}
+
+// Stages s0 .. s1-1 of f out of LDS (ping-pong a -> b); returns the buffer
+// holding the result.
+template <typename T>
+__device__ C2<T> *run_stages(const FftDesc &f, int s0, int s1, C2<T> *a, C2<T> *b, int L,
+                             int ld, const C2<T> *__restrict__ tw,
+                             const C2<T> *__restrict__ gen, T dir) {
+  for (int s = s0; s < s1; ++s) {
+    const Stage st = f.st[s];
+    const C2<T> *t = tw + st.twoff;
+    const C2<T> *g = gen + st.genoff;
+    switch (st.radix) {
+      case 2: stage_fixed<2>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 3: stage_fixed<3>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 4: stage_fixed<4>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 5: stage_fixed<5>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 7: stage_fixed<7>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 8: stage_fixed<8>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 9: stage_fixed<9>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 11: stage_fixed<11>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 13: stage_fixed<13>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 17: stage_fixed<17>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      case 16: stage_fixed<16>(a, b, L, ld, f.n, st.ns, t, g, dir); break;
+      default: stage_generic(a, b, L, ld, f.n, st.radix, st.ns, t, g, dir); break;
+    }
+    __syncthreads();
+    C2<T> *x = a;
+    a = b;
+    b = x;
+  }
+  return a;
+}
+
 // One tile (a.L lines starting at line tile * a.L) of a line set: load with
 // the op's pre-processing, all Stockham stages out of LDS (two buffers in
-// smem_raw), store with the op's post-processing.  s_oi / s_oo / s_aux are
-// a.L-entry LDS scratch for the per-line offsets.
+// smem_raw), store with the op's post-processing.  When the first (last)
+// stage has a fixed radix it reads (writes) global memory directly instead
+// of going through LDS; ops whose store needs two bins at once (r2c of even
+// length, the real-to-real modes) and the Bluestein ops keep the LDS store.
+// s_oi / s_oo / s_aux are a.L-entry LDS scratch for the per-line offsets.
 template <typename T, int OP, bool VIN, bool VOUT>
 __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
                                           unsigned char *smem_raw, int64_t *s_oi,
@@ -713,8 +825,8 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
 
   const int ld = a.ld;
   const bool conjf = (a.sub & SUB_CONJ) != 0;
-  // ---------------------------------------------------------------- load
-  for_tile(nl, a.nin, a.lfast_in, a.log2L, [&](int l, int j) {
+  // ---------------------------------------------------- per-element load
+  auto load = [&](int l, int j) -> C2<T> {
     const T *base = a.in + s_oi[l];
     C2<T> v;
     if constexpr (OP == OP_C2C || OP == OP_PASSB || OP == OP_BLUEROW) {
@@ -774,12 +886,91 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
         if (!conjf) v.y = -v.y;
       }
     }
-    bufA[l * ld + j] = v;
-  });
-  __syncthreads();
+    return v;
+  };
+  // -------------------------------------- per-element store (bin k < nout)
+  constexpr bool ELEM_STORE = OP == OP_C2C || OP == OP_PASSA || OP == OP_PASSB ||
+                              OP == OP_R2C_ODD || OP == OP_C2R_EVEN || OP == OP_C2R_ODD;
+  auto store = [&](int l, int k, C2<T> v) {
+    if (k >= a.nout) return;
+    T *base = a.out + s_oo[l];
+    if constexpr (OP == OP_C2C) {
+      stc<VOUT>(base + k * a.eso, a.fct * v);
+    } else if constexpr (OP == OP_PASSA) {
+      if (a.sub & SUB_FINAL) {  // Bluestein columns, last pass: chirp and fct, m < n
+        const int64_t m = s_aux[l] + int64_t(k) * a.other;
+        if (m < a.bign) {
+          const C2<T> c = a.t0[m];
+          stc<VOUT>(base + k * a.eso, a.fct * cmul(v, conjf ? c : conjc(c)));
+        }
+      } else {
+        const C2<T> w = dirw(big_twiddle(a, int64_t(k) * s_aux[l]), a.dir);
+        stc<VOUT>(base + k * a.eso, cmul(v, w));
+      }
+    } else if constexpr (OP == OP_BLUEROW) {
+      const C2<T> w = dirw(big_twiddle(a, int64_t(k) * s_aux[l]), T(1));
+      stc<VOUT>(base + k * a.eso, cmul(v, w));
+    } else if constexpr (OP == OP_PASSB) {
+      const int64_t m = s_aux[l] + int64_t(k) * a.other;
+      if (a.sub & SUB_KERNEL) {
+        const C2<T> w = a.t1[m];
+        stc<VOUT>(base + k * a.eso, cmul(v, conjf ? conjc(w) : w));
+      } else if (a.sub & SUB_FINAL) {
+        if (m < a.bign) {
+          const C2<T> c = a.t0[m];
+          stc<VOUT>(base + k * a.eso, a.fct * cmul(v, conjf ? c : conjc(c)));
+        }
+      } else {
+        stc<VOUT>(base + k * a.eso, a.fct * v);
+      }
+    } else if constexpr (OP == OP_BLUE) {
+      const C2<T> c = a.t0[k];
+      stc<VOUT>(base + k * a.eso, a.fct * cmul(v, conjf ? c : conjc(c)));
+    } else if constexpr (OP == OP_R2C_ODD) {
+      if (conjf) v.y = -v.y;
+      stc<VOUT>(base + k * a.eso, a.fct * v);
+    } else if constexpr (OP == OP_C2R_EVEN) {
+      const C2<T> z = a.fct * v;
+      if constexpr (VOUT) {  // contiguous real line: one 2-wide store
+        stc<true>(base + 2 * k, z);
+      } else {
+        base[2 * k * a.eso] = z.x;
+        base[(2 * k + 1) * a.eso] = z.y;
+      }
+    } else if constexpr (OP == OP_C2R_ODD) {
+      base[k * a.eso] = a.fct * v.x;
+    }
+  };
 
-  // ----------------------------------------------------------------- FFT
-  C2<T> *res = run_fft<T>(a.f, bufA, bufB, a.L, ld, a.tw, a.gen, a.dir);
+  // ---------------------------------------------------------- load + FFT
+  const FftDesc &f = a.f;
+  const bool two_ffts = OP == OP_BLUE || OP == OP_BLUEROW;
+  // the real-to-real modes keep their (heavier) loads out of the fused stage
+  const bool fuse_in = OP != OP_R2R && f.nst >= 1 && fusable_radix(f.st[0].radix);
+  // the last stage writes global memory itself when it is not also the
+  // fused first stage and the op stores bin by bin
+  const bool fuse_out = ELEM_STORE && !two_ffts && f.nst >= 2 &&
+                        fusable_radix(f.st[f.nst - 1].radix);
+  C2<T> *res;
+  if (fuse_in) {
+    stage0_in_any(f.st[0].radix, load, bufA, a.L, nl, ld, f.n, a.lfast_in, a.log2L,
+                  a.gen + f.st[0].genoff, a.dir);
+    __syncthreads();
+    res = run_stages<T>(f, 1, fuse_out ? f.nst - 1 : f.nst, bufA, bufB, a.L, ld, a.tw, a.gen,
+                        a.dir);
+  } else {
+    for_tile(nl, a.nin, a.lfast_in, a.log2L,
+             [&](int l, int j) { bufA[l * ld + j] = load(l, j); });
+    __syncthreads();
+    res = run_stages<T>(f, 0, fuse_out ? f.nst - 1 : f.nst, bufA, bufB, a.L, ld, a.tw, a.gen,
+                        a.dir);
+  }
+  if (fuse_out) {
+    const Stage st = f.st[f.nst - 1];
+    stage_last_out_any(st.radix, store, res, a.L, nl, ld, st.ns, a.lfast_out, a.log2L,
+                       a.tw + st.twoff, a.gen + st.genoff, a.dir);
+    return;
+  }
   if constexpr (OP == OP_BLUE || OP == OP_BLUEROW) {
     const int n2 = a.f.n;
     for_tile(nl, n2, 0, a.log2L, [&](int l, int k) {
@@ -787,46 +978,15 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
       res[l * ld + k] = cmul(res[l * ld + k], conjf ? conjc(w) : w);
     });
     __syncthreads();
-    res = run_fft<T>(a.f, res, res == bufA ? bufB : bufA, a.L, ld, a.tw, a.gen, -a.dir);
+    res = run_stages<T>(f, 0, f.nst, res, res == bufA ? bufB : bufA, a.L, ld, a.tw, a.gen,
+                        -a.dir);
   }
 
   // --------------------------------------------------------------- store
   for_tile(nl, a.nout, a.lfast_out, a.log2L, [&](int l, int k) {
-    T *base = a.out + s_oo[l];
     const C2<T> *r = res + l * ld;
-    if constexpr (OP == OP_C2C) {
-      stc<VOUT>(base + k * a.eso, a.fct * r[k]);
-    } else if constexpr (OP == OP_PASSA) {
-      if (a.sub & SUB_FINAL) {  // Bluestein columns, last pass: chirp and fct, m < n
-        const int64_t m = s_aux[l] + int64_t(k) * a.other;
-        if (m < a.bign) {
-          const C2<T> c = a.t0[m];
-          stc<VOUT>(base + k * a.eso, a.fct * cmul(r[k], conjf ? c : conjc(c)));
-        }
-      } else {
-        const C2<T> w = dirw(big_twiddle(a, int64_t(k) * s_aux[l]), a.dir);
-        stc<VOUT>(base + k * a.eso, cmul(r[k], w));
-      }
-    } else if constexpr (OP == OP_BLUEROW) {
-      const C2<T> w = dirw(big_twiddle(a, int64_t(k) * s_aux[l]), T(1));
-      stc<VOUT>(base + k * a.eso, cmul(r[k], w));
-    } else if constexpr (OP == OP_PASSB) {
-      const int64_t m = s_aux[l] + int64_t(k) * a.other;
-      if (a.sub & SUB_KERNEL) {
-        const C2<T> w = a.t1[m];
-        stc<VOUT>(base + k * a.eso, cmul(r[k], conjf ? conjc(w) : w));
-      } else if (a.sub & SUB_FINAL) {
-        if (m < a.bign) {
-          const C2<T> c = a.t0[m];
-          stc<VOUT>(base + k * a.eso, a.fct * cmul(r[k], conjf ? c : conjc(c)));
-        }
-      } else {
-        stc<VOUT>(base + k * a.eso, a.fct * r[k]);
-      }
-    } else if constexpr (OP == OP_BLUE) {
-      const C2<T> c = a.t0[k];
-      stc<VOUT>(base + k * a.eso, a.fct * cmul(r[k], conjf ? c : conjc(c)));
-    } else if constexpr (OP == OP_R2C_EVEN) {
+    if constexpr (OP == OP_R2C_EVEN) {
+      T *base = a.out + s_oo[l];
       const int h = a.f.n;
       C2<T> X;
       if (k == 0 || k == h) {
@@ -841,24 +1001,13 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
       }
       if (conjf) X.y = -X.y;
       stc<VOUT>(base + k * a.eso, a.fct * X);
-    } else if constexpr (OP == OP_R2C_ODD) {
-      C2<T> X = r[k];
-      if (conjf) X.y = -X.y;
-      stc<VOUT>(base + k * a.eso, a.fct * X);
-    } else if constexpr (OP == OP_C2R_EVEN) {
-      const C2<T> z = a.fct * r[k];
-      if constexpr (VOUT) {  // contiguous real line: one 2-wide store
-        stc<true>(base + 2 * k, z);
-      } else {
-        base[2 * k * a.eso] = z.x;
-        base[(2 * k + 1) * a.eso] = z.y;
-      }
     } else if constexpr (OP == OP_R2R) {
+      T *base = a.out + s_oo[l];
       base[k * a.eso] = a.fct * post_elem<T>((a.sub >> 8) & 0xff, [&](int64_t q) { return r[q]; },
                                              a.bign, k, (a.sub & R2R_ORTHO) != 0,
                                              (a.sub & R2R_FLIP) != 0);
-    } else {  // OP_C2R_ODD
-      base[k * a.eso] = a.fct * r[k].x;
+    } else {
+      store(l, k, r[k]);
     }
   });
 }
