@@ -362,6 +362,8 @@ struct LineArgs {
   T *out;
   int L, log2L, ld, nin, nout, lfast_in, lfast_out;
   int auxdim;    // batch dim whose index is a line's aux (four-step row / column)
+  int twlds_lo, twlds_hi;  // pass A of the specialised kernel: four-step twiddle
+                           // tables (t0 / t1 entries) copied to LDS; 0 = read globally
   FftDesc f;
   const C2<T> *tw, *gen;
   T dir, fct;

@@ -58,6 +58,7 @@ constexpr int pitch_of() {
 template <typename T>
 struct FCtx {
   const LineArgs<T> *a;
+  const C2<T> *twl; // four-step twiddle tables in LDS (pass A), or null
   C2<T> *line;     // this thread's line in LDS
   const T *gin;    // this thread's line in global memory (input)
   T *gout;         // ... output
@@ -101,9 +102,21 @@ __device__ __forceinline__ C2<T> fload(const FCtx<T> &c, int p) {
   }
 }
 
+// four-step twiddle W_N^e from the LDS copy of the lo / hi tables, or from
+// global memory
+template <typename T>
+__device__ __forceinline__ C2<T> ftwiddle(const LineArgs<T> &a, const C2<T> *twl, int64_t e) {
+  if (twl) {
+    const int64_t mask = (int64_t(1) << a.shift) - 1;
+    return cmul(twl[e & mask], twl[a.twlds_lo + (e >> a.shift)]);
+  }
+  return big_twiddle(a, e);
+}
+
 // elementwise store of bin p of this thread's line (all ops except r2c)
 template <typename T, int OP, bool VOUT>
-__device__ __forceinline__ void fstore(const LineArgs<T> &a, T *gout, int aux, int p, C2<T> v) {
+__device__ __forceinline__ void fstore(const LineArgs<T> &a, const C2<T> *twl, T *gout, int aux,
+                                       int p, C2<T> v) {
   if constexpr (OP == OP_C2C) {
     stc<VOUT>(gout + p * a.eso, a.fct * v);
   } else if constexpr (OP == OP_PASSA) {
@@ -114,7 +127,7 @@ __device__ __forceinline__ void fstore(const LineArgs<T> &a, T *gout, int aux, i
         stc<VOUT>(gout + p * a.eso, a.fct * cmul(v, (a.sub & SUB_CONJ) ? ch : conjc(ch)));
       }
     } else {
-      const C2<T> w = dirw(big_twiddle(a, int64_t(p) * aux), a.dir);
+      const C2<T> w = dirw(ftwiddle(a, twl, int64_t(p) * aux), a.dir);
       stc<VOUT>(gout + p * a.eso, cmul(v, w));
     }
   } else if constexpr (OP == OP_PASSB) {
@@ -201,7 +214,8 @@ __device__ __forceinline__ void fstage(const FCtx<T> &c) {
         const int j = c.i + q * TPL;
         if (Q * TPL == NR || j < NR) {
 #pragma unroll
-          for (int r = 0; r < R; ++r) fstore<T, OP, VOUT>(a, c.gout, c.aux, j + r * NS, v[q][r]);
+          for (int r = 0; r < R; ++r)
+            fstore<T, OP, VOUT>(a, c.twl, c.gout, c.aux, j + r * NS, v[q][r]);
         }
       }
     }
@@ -239,6 +253,17 @@ __global__ void __launch_bounds__(TPL *LPB) k_fast(const LineArgs<T> a) {
   const int64_t b = tile * LPB + l;
   FCtx<T> c;
   c.a = &a;
+  c.twl = nullptr;
+  if constexpr (OP == OP_PASSA) {
+    if (a.twlds_lo > 0) {  // copy the four-step twiddle tables behind the lines
+      C2<T> *twl = lds + LPB * P;
+      const int nt = a.twlds_lo + a.twlds_hi;
+      for (int q = t; q < nt; q += TPL * LPB)
+        twl[q] = q < a.twlds_lo ? a.t0[q] : a.t1[q - a.twlds_lo];
+      __syncthreads();
+      c.twl = twl;
+    }
+  }
   c.line = lds + l * P;
   c.valid = b < a.nlines;
   c.i = i;
@@ -278,7 +303,7 @@ __global__ void __launch_bounds__(TPL *LPB) k_fast(const LineArgs<T> a) {
         for (int k = io; k <= N; k += TPL) stc<VOUT>(g + k * a.eso, r2c_bin(a, ln, k));
       } else {
         for (int k = io; k < N; k += TPL)
-          fstore<T, OP, VOUT>(a, g, s_aux[lo], k, ln[padi<T>(k)]);
+          fstore<T, OP, VOUT>(a, c.twl, g, s_aux[lo], k, ln[padi<T>(k)]);
       }
     }
   }
@@ -291,10 +316,35 @@ std::set<const void *> g_fast_attr;
 
 template <typename T, int N, int TPL, int LPB, bool LF_IN, bool LF_OUT, int OP, bool VIN,
           bool VOUT, typename SC>
-void go(const LineArgs<T> &a, hipStream_t s) {
+void go(const LineArgs<T> &a0, hipStream_t s) {
   const void *fn = reinterpret_cast<const void *>(
       &k_fast<T, N, TPL, LPB, LF_IN, LF_OUT, OP, VIN, VOUT, SC>);
-  const size_t lds = size_t(LPB) * pitch_of<T, N>() * sizeof(C2<T>);
+  size_t lds = size_t(LPB) * pitch_of<T, N>() * sizeof(C2<T>);
+  LineArgs<T> a = a0;
+  a.twlds_lo = a.twlds_hi = 0;
+  if constexpr (OP == OP_PASSA) {
+    // pass A's four-step twiddle tables go to LDS when they fit beside the
+    // lines without lowering the number of workgroups per CU (the column
+    // stores gather them at scattered indices)
+    if (!(a.sub & SUB_FINAL)) {
+      const int64_t big = int64_t(N) * a.other;
+      const int64_t lo_n = int64_t(1) << a.shift, hi_n = (big + lo_n - 1) / lo_n;
+      const size_t extra = size_t(lo_n + hi_n) * sizeof(C2<T>);
+      static const size_t static_lds = [fn] {
+        hipFuncAttributes fa;
+        HIP_CHECK(hipFuncGetAttributes(&fa, fn));
+        return size_t(fa.sharedSizeBytes);
+      }();
+      // only where the copy costs no workgroup slot per CU
+      const size_t cap = lds_limit();
+      if (lds + extra + static_lds <= cap &&
+          cap / (lds + extra + static_lds) == cap / (lds + static_lds)) {
+        a.twlds_lo = int(lo_n);
+        a.twlds_hi = int(hi_n);
+        lds += extra;
+      }
+    }
+  }
   if (lds > 65536) {
     std::lock_guard<std::mutex> lk(g_fast_mu);
     if (!g_fast_attr.count(fn)) {
