@@ -44,9 +44,9 @@ BYTES_C2C_PASS = 2 * N3 * N3 * (N3 // 2 + 1) * 8
 #   rocprofv3 --pmc FETCH_SIZE -- python3 bench.py --probe   (then WRITE_SIZE)
 # each counter scaled by what it reads for the device copy of known size
 # (67,633,152 B each way) in the same run: FETCH_SIZE x 1.999, WRITE_SIZE
-# x 1.000 on MI355X.  Mean of six launches: 67,781,931 B fetched +
-# 69,142,811 B written = 1.012 x the algorithmic bytes.
-TRAFFIC_C2C_PASS = 136924742
+# x 1.000 on MI355X.  Mean of six launches: 67,767,836 B fetched +
+# 68,723,963 B written = 1.009 x the algorithmic bytes.
+TRAFFIC_C2C_PASS = 136491799
 
 
 def rel_l2(a, b):
