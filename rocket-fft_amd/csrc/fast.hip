@@ -312,7 +312,7 @@ __global__ void __launch_bounds__(TPL *LPB) k_fast(const LineArgs<T> a) {
 // ------------------------------------------------------------ dispatch
 
 std::mutex g_fast_mu;
-std::set<const void *> g_fast_attr;
+std::set<std::pair<int, const void *>> g_fast_attr;  // (device, kernel)
 
 template <typename T, int N, int TPL, int LPB, bool LF_IN, bool LF_OUT, int OP, bool VIN,
           bool VOUT, typename SC>
@@ -346,13 +346,15 @@ void go(const LineArgs<T> &a0, hipStream_t s) {
     }
   }
   if (lds > 65536) {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_fast_mu);
-    if (!g_fast_attr.count(fn)) {
+    if (!g_fast_attr.count({dev, fn})) {
       hipFuncAttributes fa;
       HIP_CHECK(hipFuncGetAttributes(&fa, fn));
       HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     int(lds_limit() - fa.sharedSizeBytes)));
-      g_fast_attr.insert(fn);
+      g_fast_attr.insert({dev, fn});
     }
   }
   const int64_t groups = (a.nlines + LPB - 1) / LPB;

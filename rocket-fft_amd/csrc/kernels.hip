@@ -81,14 +81,16 @@ bool pairs_aligned(const void *p, size_t scalar, int64_t es, const int64_t *bs, 
 }
 
 std::mutex g_attr_mu;
-std::set<const void *> g_attr_done;
+std::set<std::pair<int, const void *>> g_attr_done;
 
 // Raises a kernel's dynamic-LDS ceiling to what the device allows beside the
-// kernel's static LDS (once per kernel).
+// kernel's static LDS (once per kernel and device).
 void allow_lds(const void *fn, size_t bytes) {
   if (bytes <= 65536) return;
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_attr_mu);
-  if (g_attr_done.count(fn)) return;
+  if (g_attr_done.count({dev, fn})) return;
   hipFuncAttributes fa;
   HIP_CHECK(hipFuncGetAttributes(&fa, fn));
   const int dyn = int(lds_limit()) - int(fa.sharedSizeBytes);
@@ -100,7 +102,7 @@ void allow_lds(const void *fn, size_t bytes) {
   if (e != hipSuccess)
     RFH_THROW(2, std::string("hipFuncSetAttribute(MaxDynamicSharedMemorySize, ") +
                      std::to_string(dyn) + "): " + hipGetErrorString(e));
-  g_attr_done.insert(fn);
+  g_attr_done.insert({dev, fn});
 }
 
 template <typename T, int OP, bool VIN, bool VOUT>

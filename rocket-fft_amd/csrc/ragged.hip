@@ -20,6 +20,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <set>
 
 #include "../../include/rocket_fft_hip.h"
 #include "passes.hpp"
@@ -282,7 +283,7 @@ std::unique_ptr<Built> build(const std::vector<rfh_job_t> &jobs, bool dbl, size_
 }
 
 std::mutex g_attr_mu;
-bool g_attr_set[2] = {false, false};
+std::set<std::pair<int, int>> g_attr_set;  // (device, double precision)
 
 template <typename T>
 void launch_batch(const Built &B, int grid, hipStream_t s) {
@@ -290,12 +291,14 @@ void launch_batch(const Built &B, int grid, hipStream_t s) {
   const void *fn = reinterpret_cast<const void *>(&k_ragged<T>);
   {
     std::lock_guard<std::mutex> lk(g_attr_mu);
-    if (!g_attr_set[sizeof(T) == 8]) {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (!g_attr_set.count({dev, int(sizeof(T) == 8)})) {
       hipFuncAttributes fa;
       HIP_CHECK(hipFuncGetAttributes(&fa, fn));
       HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     int(lds_limit() - fa.sharedSizeBytes)));
-      g_attr_set[sizeof(T) == 8] = true;
+      g_attr_set.insert({dev, int(sizeof(T) == 8)});
     }
   }
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, static_cast<unsigned *>(B.queue.p), 1);
