@@ -119,8 +119,8 @@ int rfh_set_fast_path(bool enabled);
  * hi/lo splits only (precision study; about 1e-5).  No reference
  * counterpart.  Returns the previous mode. */
 int rfh_set_precision(int mode);
-/* One 16x16x16 matrix-core product on device buffers laid out as the
- * hardware operands (a, b: 64 lanes x 4 floats, d: 64 x 4; bf16 != 0 selects
+/* One 16x16x32 matrix-core product on device buffers laid out as the
+ * hardware operands (a, b: 64 lanes x 8 floats, d: 64 x 4; bf16 != 0 selects
  * BF16, else FP16), for tests of the operand layout. */
 int rfh_mfma_probe(int bf16, const float *a, const float *b, float *d, void *stream);
 

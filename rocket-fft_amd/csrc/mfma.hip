@@ -2,12 +2,14 @@
 // complex matrix products on the gfx950 matrix cores.
 //
 // A radix-16 butterfly is X = F v with F[k][r] = e^{dir 2 pi i k r / 16}.
-// Sixteen butterflies of a wave are one complex matrix product D = F B
-// (B: one column per butterfly), done as four real products
-//   D_re = F_re B_re + (-F_im) B_im,      D_im = F_im B_re + F_re B_im
-// with v_mfma_f32_16x16x16_{f16,bf16} and FP32 accumulation.  Each lane
-// holds four inputs of one butterfly, both parts, so every value is loaded
-// and twiddled once.
+// Sixteen butterflies of a wave are one matrix product D (16x16) = A (16x32)
+// x B (32x16) with the complex product embedded in real arithmetic:
+//   B = [Re v; Im v] (one column per butterfly),
+//   A_re = [Re F, -Im F]  ->  D_re = Re X,     A_im = [Im F, Re F]  ->  D_im = Im X,
+// on v_mfma_f32_16x16x32_{f16,bf16} with FP32 accumulation.  The four lanes
+// of a butterfly each load four of its sixteen inputs (both parts) once;
+// lanes l and l + 32 then swap the halves the operand layout wants
+// (v_permlane32_swap), so no input is loaded or twiddled twice.
 // FP16 form: every operand is split x = hi + lo (two FP16 values, ~22 bits),
 // and D accumulates hi*hi + hi*lo + lo*hi; each butterfly is first scaled by
 // a power of two so its largest input is just below 2^10 (exact, undone
@@ -17,8 +19,8 @@
 // accumulated; a two-way BF16 split (three products, ~16 bits) is kept as a
 // precision-study mode.
 //
-// Operand layout of v_mfma_f32_16x16x16_{f16,bf16} (64 lanes; lane l, g =
-// l / 16): A row l % 16, columns 4g..4g+3; B rows 4g..4g+3, column l % 16;
+// Operand layout of v_mfma_f32_16x16x32_{f16,bf16} (64 lanes; lane l, g =
+// l / 16): A row l % 16, columns 8g..8g+7; B rows 8g..8g+7, column l % 16;
 // D rows 4g..4g+3, column l % 16.  k_mfma_probe lets a GPU test check this
 // layout against a host matrix product.
 //
@@ -35,31 +37,31 @@ int g_precision_mode = 0;  // 0 plain FP32, 1 FP16 split, 2 BF16 3-split, 3 BF16
 namespace {
 using namespace dev;
 
-typedef _Float16 half4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf164 __attribute__((ext_vector_type(4)));
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf168 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 enum { MODE_F16 = 1, MODE_BF16_3 = 2, MODE_BF16_2 = 3 };
 
 __global__ void k_mfma_probe(int bf, const float *a, const float *b, float *d) {
-  // a, b: 64 lanes x 4 values in the operand layout above; d: 64 x 4
+  // a, b: 64 lanes x 8 values in the operand layout above; d: 64 x 4
   const int l = threadIdx.x;
   f4 c = {0.f, 0.f, 0.f, 0.f};
   f4 r;
   if (bf) {
-    bf164 x, y;
-    for (int i = 0; i < 4; ++i) {
-      x[i] = (__bf16)a[l * 4 + i];
-      y[i] = (__bf16)b[l * 4 + i];
+    bf168 x, y;
+    for (int i = 0; i < 8; ++i) {
+      x[i] = (__bf16)a[l * 8 + i];
+      y[i] = (__bf16)b[l * 8 + i];
     }
-    r = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(x, y, c, 0, 0, 0);
+    r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, y, c, 0, 0, 0);
   } else {
-    half4 x, y;
-    for (int i = 0; i < 4; ++i) {
-      x[i] = (_Float16)a[l * 4 + i];
-      y[i] = (_Float16)b[l * 4 + i];
+    half8 x, y;
+    for (int i = 0; i < 8; ++i) {
+      x[i] = (_Float16)a[l * 8 + i];
+      y[i] = (_Float16)b[l * 8 + i];
     }
-    r = __builtin_amdgcn_mfma_f32_16x16x16f16(x, y, c, 0, 0, 0);
+    r = __builtin_amdgcn_mfma_f32_16x16x32_f16(x, y, c, 0, 0, 0);
   }
   for (int i = 0; i < 4; ++i) d[l * 4 + i] = r[i];
 }
@@ -69,30 +71,30 @@ struct Ops;
 
 template <>
 struct Ops<MODE_F16> {
-  typedef half4 V;
+  typedef half8 V;
   static constexpr int terms = 2;
   __device__ static V split(const float *x, int t) {
     V v;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 8; ++i) {
       const _Float16 h = (_Float16)x[i];
       v[i] = t == 0 ? h : (_Float16)(x[i] - (float)h);
     }
     return v;
   }
   __device__ static f4 mma(V a, V b, f4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
   }
 };
 
 template <int TERMS>
 struct OpsBf {
-  typedef bf164 V;
+  typedef bf168 V;
   static constexpr int terms = TERMS;
   __device__ static V split(const float *x, int t) {
     V v;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 8; ++i) {
       float r = x[i];
       __bf16 p = (__bf16)r;
       for (int s = 0; s < t; ++s) {
@@ -104,7 +106,7 @@ struct OpsBf {
     return v;
   }
   __device__ static f4 mma(V a, V b, f4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
 };
 template <>
@@ -130,31 +132,34 @@ __global__ void __launch_bounds__(256, 4) k_fft4096_mfma(const LineArgs<float> a
   __shared__ C2<float> line[N + (N >> 4) + 1];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col = lane & 15, kg = lane >> 4;
+  const bool upper = kg >= 2;  // lanes 32-63
+  // this lane's four inputs (of sixteen) per butterfly: lanes kg and kg ^ 2
+  // share inputs (kg & 1) * 8 .. + 7, the lower lane loading the first four
+  const int r0 = (kg & 1) * 8 + (upper ? 4 : 0);
   const float dir = a.dir;
   const int64_t b = xcd_tile();
   const bool valid = b < a.nlines;
   const float *gin = a.in + b * a.bsi[a.nb > 0 ? a.nb - 1 : 0];
   float *gout = a.out + b * a.bso[a.nb > 0 ? a.nb - 1 : 0];
 
-  // A operands: row = output k (lane % 16), columns r = 4 kg .. 4 kg + 3:
-  // F_re, F_im and -F_im, each split into terms
-  V Fr[O::terms], Fi[O::terms], Fn[O::terms];
+  // A operands: row = output k (lane % 16), columns 8 kg .. 8 kg + 7 of
+  // [Re | Im]: A_re = [Re F, -Im F], A_im = [Im F, Re F]
+  V Are[O::terms], Aim[O::terms];
   {
-    float re[4], im[4], ne[4];
+    float re[8], im[8];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int r = 4 * kg + i;
+    for (int i = 0; i < 8; ++i) {
+      const int kk = 8 * kg + i, r = kk & 15;
       float sn, cs;
       sincospif(float((col * r) & 15) / 8.0f, &sn, &cs);  // angle 2 pi k r / 16
-      re[i] = cs;
-      im[i] = dir * sn;
-      ne[i] = -im[i];
+      const float fr = cs, fi = dir * sn;
+      re[i] = kk < 16 ? fr : -fi;
+      im[i] = kk < 16 ? fi : fr;
     }
 #pragma unroll
     for (int t = 0; t < O::terms; ++t) {
-      Fr[t] = O::split(re, t);
-      Fi[t] = O::split(im, t);
-      Fn[t] = O::split(ne, t);
+      Are[t] = O::split(re, t);
+      Aim[t] = O::split(im, t);
     }
   }
 
@@ -173,7 +178,7 @@ __global__ void __launch_bounds__(256, 4) k_fft4096_mfma(const LineArgs<float> a
       const int k = j % NS;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int r = 4 * kg + i;
+        const int r = r0 + i;
         if (s == 0)
           in[g][i] = valid ? ldc<true>(gin + 2 * (j + r * NR)) : C2<float>{0.f, 0.f};
         else
@@ -183,32 +188,34 @@ __global__ void __launch_bounds__(256, 4) k_fft4096_mfma(const LineArgs<float> a
     }
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-      float xr[4], xi[4];
       float m = 0.f;
+      C2<float> v[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        C2<float> v = in[g][i];
-        if (s > 0) v = cmul(v, dirw(w[g][i], dir));
-        xr[i] = v.x;
-        xi[i] = v.y;
-        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+        v[i] = in[g][i];
+        if (s > 0) v[i] = cmul(v[i], dirw(w[g][i], dir));
+        m = fmaxf(m, fmaxf(fabsf(v[i].x), fabsf(v[i].y)));
       }
       m = group_max(m);
       int ex = 0;
       (void)frexpf(m, &ex);
       const float sc = m > 0.f ? ldexpf(1.f, 10 - ex) : 1.f;
       inv[g] = m > 0.f ? ldexpf(1.f, ex - 10) : 1.f;  // exact: sc is a power of two
+      // B rows 8 kg .. 8 kg + 7: the lower lane (l < 32) takes the real
+      // parts of the eight shared inputs (its own four, then those of lane
+      // l + 32), the upper lane the imaginary parts (those of lane l - 32,
+      // then its own); one v_permlane32_swap of (Re, Im) yields exactly that
+      float x[8];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        xr[i] *= sc;
-        xi[i] *= sc;
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[i].x * sc),
+                                                        __float_as_uint(v[i].y * sc), false, false);
+        x[i] = __uint_as_float(r[0]);
+        x[4 + i] = __uint_as_float(r[1]);
       }
-      V Br[O::terms], Bi[O::terms];
+      V B[O::terms];
 #pragma unroll
-      for (int t = 0; t < O::terms; ++t) {
-        Br[t] = O::split(xr, t);
-        Bi[t] = O::split(xi, t);
-      }
+      for (int t = 0; t < O::terms; ++t) B[t] = O::split(x, t);
       f4 zr = {0.f, 0.f, 0.f, 0.f}, zi = {0.f, 0.f, 0.f, 0.f};
       // products down to the split's precision: sum of term indices < terms
 #pragma unroll
@@ -216,10 +223,8 @@ __global__ void __launch_bounds__(256, 4) k_fft4096_mfma(const LineArgs<float> a
 #pragma unroll
         for (int tb = 0; tb < O::terms; ++tb)
           if (ta + tb < O::terms) {
-            zr = O::mma(Fr[ta], Br[tb], zr);
-            zr = O::mma(Fn[ta], Bi[tb], zr);
-            zi = O::mma(Fi[ta], Br[tb], zi);
-            zi = O::mma(Fr[ta], Bi[tb], zi);
+            zr = O::mma(Are[ta], B[tb], zr);
+            zi = O::mma(Aim[ta], B[tb], zi);
           }
       dre[g] = zr;
       dim[g] = zi;

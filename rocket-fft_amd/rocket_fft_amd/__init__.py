@@ -327,8 +327,8 @@ def set_precision(mode):
 
 
 def mfma_probe(a_lanes, b_lanes, bf16=False):
-    """d (64 x 4) = one 16x16x16 matrix-core product of device operands laid
-    out per lane (64 x 4 each), float32 torch CUDA tensors."""
+    """d (64 x 4) = one 16x16x32 matrix-core product of device operands laid
+    out per lane (64 x 8 each), float32 torch CUDA tensors."""
     import torch
     L = _load()
     d = torch.zeros((64, 4), dtype=torch.float32, device=a_lanes.device)

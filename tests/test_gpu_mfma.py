@@ -1,6 +1,6 @@
 """Split-precision form: radix-16 stages on the matrix cores.
 
-* The operand layout the kernels assume for v_mfma_f32_16x16x16_{f16,bf16}
+* The operand layout the kernels assume for v_mfma_f32_16x16x32_{f16,bf16}
   is checked exactly with small integers (exact in FP16/BF16/FP32).
 * complex64 transforms of length 4096 in each precision mode against the
   long-double oracle on both input sets of the precision configuration:
@@ -19,14 +19,14 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("bf16", [False, True])
 def test_mfma_operand_layout(bf16):
     rs = np.random.RandomState(1)
-    A = rs.randint(-4, 5, (16, 16)).astype(np.float32)
-    B = rs.randint(-4, 5, (16, 16)).astype(np.float32)
-    al = np.zeros((64, 4), np.float32)
-    bl = np.zeros((64, 4), np.float32)
+    A = rs.randint(-4, 5, (16, 32)).astype(np.float32)
+    B = rs.randint(-4, 5, (32, 16)).astype(np.float32)
+    al = np.zeros((64, 8), np.float32)
+    bl = np.zeros((64, 8), np.float32)
     for l in range(64):
         g = l // 16
-        al[l] = A[l % 16, 4 * g:4 * g + 4]
-        bl[l] = B[4 * g:4 * g + 4, l % 16]
+        al[l] = A[l % 16, 8 * g:8 * g + 8]
+        bl[l] = B[8 * g:8 * g + 8, l % 16]
     d = R.mfma_probe(torch.from_numpy(al).cuda(), torch.from_numpy(bl).cuda(), bf16).cpu().numpy()
     D = A @ B
     want = np.zeros((64, 4), np.float32)
