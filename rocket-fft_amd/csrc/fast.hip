@@ -60,6 +60,11 @@ struct FCtx {
   const LineArgs<T> *a;
   const C2<T> *twl; // four-step twiddle tables in LDS (pass A), or null
   C2<T> *line;     // this thread's line in LDS
+  T *sline;        // ... in the split layout (one part at a time)
+  T *soline;       // line of the output mapping in the split layout
+  const int64_t *s_oo;  // per-line output offsets (store through LDS)
+  const int *s_aux;
+  int lo, io, nl;  // output mapping: line, index within it, lines in the tile
   const T *gin;    // this thread's line in global memory (input)
   T *gout;         // ... output
   int aux;
@@ -237,13 +242,147 @@ __device__ __forceinline__ void fstage(const FCtx<T> &c) {
   }
 }
 
+// Split exchange: the same stages, but a line crosses LDS as its real parts
+// and then its imaginary parts through a buffer of N scalars per line, so
+// that a workgroup needs half the LDS (more workgroups per CU for complex128
+// lines of 1024-4096).  The next stage's inputs are read into registers
+// between the two halves; four barriers per exchange instead of two.
+template <typename SC, int N, int TPL, int S>
+struct StageShape {
+  static constexpr int R = SC::at(S);
+  static constexpr int NS = SC::ns(S);
+  static constexpr int NR = N / R;
+  static constexpr int Q = (NR + TPL - 1) / TPL;
+};
+
+template <typename T, typename SS, int TPL, int PART>
+__device__ __forceinline__ void split_write(const FCtx<T> &c, const C2<T> (&v)[SS::Q][SS::R]) {
+#pragma unroll
+  for (int q = 0; q < SS::Q; ++q) {
+    const int j = c.i + q * TPL;
+    if (SS::Q * TPL == SS::NR || j < SS::NR) {
+      const int k = j % SS::NS;
+#pragma unroll
+      for (int r = 0; r < SS::R; ++r)
+        c.sline[padi<T>((j - k) * SS::R + k + r * SS::NS)] = PART ? v[q][r].y : v[q][r].x;
+    }
+  }
+}
+
+template <typename T, typename SS, int TPL, int PART>
+__device__ __forceinline__ void split_read(const FCtx<T> &c, C2<T> (&w)[SS::Q][SS::R]) {
+#pragma unroll
+  for (int q = 0; q < SS::Q; ++q) {
+    const int j = c.i + q * TPL;
+    if (SS::Q * TPL == SS::NR || j < SS::NR) {
+#pragma unroll
+      for (int r = 0; r < SS::R; ++r) {
+        const T x = c.sline[padi<T>(j + r * SS::NR)];
+        if (PART)
+          w[q][r].y = x;
+        else
+          w[q][r].x = x;
+      }
+    }
+  }
+}
+
+template <typename T, int N, int TPL, int OP, bool VIN, bool VOUT, bool DIRECT_OUT,
+          typename SC, int S>
+__device__ __forceinline__ void fstage_split(
+    const FCtx<T> &c, C2<T> (&v)[StageShape<SC, N, TPL, S>::Q][StageShape<SC, N, TPL, S>::R]) {
+  using SS = StageShape<SC, N, TPL, S>;
+  constexpr int R = SS::R, NS = SS::NS, NR = SS::NR, Q = SS::Q;
+  constexpr bool FIRST = S == 0;
+  constexpr bool LAST = S == SC::count - 1;
+  static_assert(OP != OP_R2C_EVEN, "r2c reads two bins per output: full exchange only");
+  const LineArgs<T> &a = *c.a;
+  const T dir = a.dir;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = c.i + q * TPL;
+    if (Q * TPL == NR || j < NR) {
+      if constexpr (FIRST) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          v[q][r] = c.valid ? fload<T, OP, VIN>(c, j + r * NR) : C2<T>{T(0), T(0)};
+      }
+      if constexpr (NS > 1) {
+        const int k = j % NS;
+        const C2<T> *tw = a.tw + a.f.st[S].twoff;
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], dirw(tw[(r - 1) * NS + k], dir));
+      }
+      dft<R>(v[q], dir, a.gen);
+    }
+  }
+  if constexpr (LAST && DIRECT_OUT) {
+    if (c.valid) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int j = c.i + q * TPL;
+        if (Q * TPL == NR || j < NR) {
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            fstore<T, OP, VOUT>(a, c.twl, c.gout, c.aux, j + r * NS, v[q][r]);
+        }
+      }
+    }
+  } else if constexpr (LAST) {
+    // natural-order result, read with the output mapping and stored
+    constexpr int NK = N / TPL;
+    C2<T> z[NK];
+    const bool on = c.lo < c.nl;
+    if constexpr (!FIRST) __syncthreads();
+    split_write<T, SS, TPL, 0>(c, v);
+    __syncthreads();
+    if (on) {
+#pragma unroll
+      for (int kk = 0; kk < NK; ++kk) z[kk].x = c.soline[padi<T>(c.io + kk * TPL)];
+    }
+    __syncthreads();
+    split_write<T, SS, TPL, 1>(c, v);
+    __syncthreads();
+    if (on) {
+#pragma unroll
+      for (int kk = 0; kk < NK; ++kk) z[kk].y = c.soline[padi<T>(c.io + kk * TPL)];
+      T *g = a.out + c.s_oo[c.lo];
+      const int aux = c.s_aux[c.lo];
+#pragma unroll
+      for (int kk = 0; kk < NK; ++kk)
+        fstore<T, OP, VOUT>(a, c.twl, g, aux, c.io + kk * TPL, z[kk]);
+    }
+  } else {
+    using SN = StageShape<SC, N, TPL, S + 1>;
+    C2<T> w[SN::Q][SN::R];
+    if constexpr (!FIRST) __syncthreads();  // every read of the last exchange is done
+    split_write<T, SS, TPL, 0>(c, v);
+    __syncthreads();
+    split_read<T, SN, TPL, 0>(c, w);
+    __syncthreads();
+    split_write<T, SS, TPL, 1>(c, v);
+    __syncthreads();
+    split_read<T, SN, TPL, 1>(c, w);
+    fstage_split<T, N, TPL, OP, VIN, VOUT, DIRECT_OUT, SC, S + 1>(c, w);
+  }
+}
+
+// LDS bytes of the lines of a workgroup (the four-step twiddle tables of
+// pass A, when copied, follow at this offset)
+template <typename T, int N, int LPB, bool SPLIT>
+constexpr size_t lines_bytes() {
+  return SPLIT ? (size_t(LPB) * pitch_of<T, N>() * sizeof(T) + 15) / 16 * 16
+               : size_t(LPB) * pitch_of<T, N>() * sizeof(C2<T>);
+}
+
 template <typename T, int N, int TPL, int LPB, bool LF_IN, bool LF_OUT, int OP, bool VIN,
-          bool VOUT, typename SC>
-__global__ void __launch_bounds__(TPL *LPB) k_fast(const LineArgs<T> a) {
+          bool VOUT, typename SC, bool SPLIT>
+__global__ void __launch_bounds__(TPL *LPB, SPLIT ? 4 : 1) k_fast(const LineArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int P = pitch_of<T, N>();
   constexpr bool DIRECT_OUT = (LF_IN == LF_OUT) && OP != OP_R2C_EVEN;
   C2<T> *lds = reinterpret_cast<C2<T> *>(smem_raw);
+  T *slds = reinterpret_cast<T *>(smem_raw);
   __shared__ int64_t s_oo[LPB];
   __shared__ int s_aux[LPB];
   const int t = threadIdx.x;
@@ -256,7 +395,7 @@ __global__ void __launch_bounds__(TPL *LPB) k_fast(const LineArgs<T> a) {
   c.twl = nullptr;
   if constexpr (OP == OP_PASSA) {
     if (a.twlds_lo > 0) {  // copy the four-step twiddle tables behind the lines
-      C2<T> *twl = lds + LPB * P;
+      C2<T> *twl = reinterpret_cast<C2<T> *>(smem_raw + lines_bytes<T, N, LPB, SPLIT>());
       const int nt = a.twlds_lo + a.twlds_hi;
       for (int q = t; q < nt; q += TPL * LPB)
         twl[q] = q < a.twlds_lo ? a.t0[q] : a.t1[q - a.twlds_lo];
@@ -265,6 +404,7 @@ __global__ void __launch_bounds__(TPL *LPB) k_fast(const LineArgs<T> a) {
     }
   }
   c.line = lds + l * P;
+  c.sline = slds + l * P;
   c.valid = b < a.nlines;
   c.i = i;
   int64_t oi = 0, oo = 0;
@@ -289,8 +429,20 @@ __global__ void __launch_bounds__(TPL *LPB) k_fast(const LineArgs<T> a) {
       s_aux[l] = aux;
     }
   }
-  fstage<T, N, TPL, OP, VIN, VOUT, DIRECT_OUT, SC, 0>(c);
-  if constexpr (!DIRECT_OUT) {
+  if constexpr (SPLIT) {
+    c.nl = int(min(int64_t(LPB), a.nlines - tile * LPB));
+    c.lo = LF_OUT ? t % LPB : t / TPL;
+    c.io = LF_OUT ? t / LPB : t % TPL;
+    c.soline = slds + c.lo * P;
+    c.s_oo = s_oo;
+    c.s_aux = s_aux;
+    using S0 = StageShape<SC, N, TPL, 0>;
+    C2<T> v0[S0::Q][S0::R];
+    fstage_split<T, N, TPL, OP, VIN, VOUT, DIRECT_OUT, SC, 0>(c, v0);
+  } else {
+    fstage<T, N, TPL, OP, VIN, VOUT, DIRECT_OUT, SC, 0>(c);
+  }
+  if constexpr (!DIRECT_OUT && !SPLIT) {
     // natural-order result of every line is in LDS; store with the output
     // mapping (the last stage ended with a barrier)
     const int nl = int(min(int64_t(LPB), a.nlines - tile * LPB));
@@ -315,11 +467,11 @@ std::mutex g_fast_mu;
 std::set<std::pair<int, const void *>> g_fast_attr;  // (device, kernel)
 
 template <typename T, int N, int TPL, int LPB, bool LF_IN, bool LF_OUT, int OP, bool VIN,
-          bool VOUT, typename SC>
+          bool VOUT, typename SC, bool SPLIT>
 void go(const LineArgs<T> &a0, hipStream_t s) {
   const void *fn = reinterpret_cast<const void *>(
-      &k_fast<T, N, TPL, LPB, LF_IN, LF_OUT, OP, VIN, VOUT, SC>);
-  size_t lds = size_t(LPB) * pitch_of<T, N>() * sizeof(C2<T>);
+      &k_fast<T, N, TPL, LPB, LF_IN, LF_OUT, OP, VIN, VOUT, SC, SPLIT>);
+  size_t lds = lines_bytes<T, N, LPB, SPLIT>();
   LineArgs<T> a = a0;
   a.twlds_lo = a.twlds_hi = 0;
   if constexpr (OP == OP_PASSA) {
@@ -358,19 +510,19 @@ void go(const LineArgs<T> &a0, hipStream_t s) {
     }
   }
   const int64_t groups = (a.nlines + LPB - 1) / LPB;
-  hipLaunchKernelGGL((k_fast<T, N, TPL, LPB, LF_IN, LF_OUT, OP, VIN, VOUT, SC>),
+  hipLaunchKernelGGL((k_fast<T, N, TPL, LPB, LF_IN, LF_OUT, OP, VIN, VOUT, SC, SPLIT>),
                      dim3(unsigned(groups)), dim3(TPL * LPB), lds, s, a);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     RFH_THROW(2, std::string("fast line kernel launch: ") + hipGetErrorString(e));
 }
 
-template <typename T, int N, int TPL, int LPB, int OP, typename SC>
+template <typename T, int N, int TPL, int LPB, int OP, typename SC, bool SPLIT>
 bool go_map(const LineArgs<T> &a, bool lf_in, bool lf_out, bool vin, bool vout,
             hipStream_t s) {
 #define RFH_GO(LI, LO, VI, VO)                                    \
   if (lf_in == LI && lf_out == LO && vin == VI && vout == VO) {   \
-    go<T, N, TPL, LPB, LI, LO, OP, VI, VO, SC>(a, s);             \
+    go<T, N, TPL, LPB, LI, LO, OP, VI, VO, SC, SPLIT>(a, s);      \
     return true;                                                  \
   }
   RFH_GO(false, false, true, true)
@@ -398,18 +550,30 @@ struct FastShape {
   }
   static constexpr int LPB_I = fits(LPB_W) ? LPB_W : LPB_8;
   static constexpr bool interleaved_ok = fits(LPB_I);
+  // split exchange (half the LDS, registers held to four waves per SIMD)
+  // where it measured faster: complex128 lines of 4096, and interleaved
+  // complex128 tiles that would otherwise take a whole CU's LDS; shorter
+  // contiguous lines lose more to the extra barriers than they gain
+  static constexpr size_t full_lds(int lpb) {
+    return size_t(lpb) * pitch_of<T, N>() * sizeof(C2<T>);
+  }
+  static constexpr bool SPLIT_C = sizeof(T) == 8 && N >= 4096 && full_lds(LPB_C) > 65536;
+  static constexpr bool SPLIT_I = sizeof(T) == 8 && full_lds(LPB_I) > 131072;
 };
 
 template <typename T, int N, int OP, typename SC>
 bool go_len(const LineArgs<T> &a, bool lf_in, bool lf_out, bool vin, bool vout,
             hipStream_t s) {
   using F = FastShape<T, N>;
+  constexpr bool r2c = OP == OP_R2C_EVEN;
   if (lf_in || lf_out) {
     if constexpr (F::interleaved_ok)
-      return go_map<T, N, F::TPL, F::LPB_I, OP, SC>(a, lf_in, lf_out, vin, vout, s);
+      return go_map<T, N, F::TPL, F::LPB_I, OP, SC, F::SPLIT_I && !r2c>(a, lf_in, lf_out, vin,
+                                                                        vout, s);
     return false;
   }
-  return go_map<T, N, F::TPL, F::LPB_C, OP, SC>(a, lf_in, lf_out, vin, vout, s);
+  return go_map<T, N, F::TPL, F::LPB_C, OP, SC, F::SPLIT_C && !r2c>(a, lf_in, lf_out, vin, vout,
+                                                                    s);
 }
 
 template <typename T>

@@ -468,7 +468,7 @@ namespace {
 // kChunkBytes, so that pass B reads what pass A just wrote from the Infinity
 // Cache (256 MB on MI355X) instead of HBM; chunking needs a single batch
 // dimension to slice.
-constexpr size_t kChunkBytes = size_t(64) << 20;
+constexpr size_t kChunkBytes = size_t(128) << 20;
 
 int64_t chunk_lines(const C2CPlan &p, const LineSet &g) {
   if (p.kind != kTwoPass && p.kind != kBlueTwoPass) return g.nlines;

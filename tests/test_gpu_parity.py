@@ -232,12 +232,15 @@ def test_fast_and_generic_paths(fast, n, dtype):
         R.set_fast_path(True)
 
 
+@pytest.mark.parametrize("n", [1 << 18, 1 << 20])
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
-def test_two_pass_pow2_fast(dtype):
+def test_two_pass_pow2_fast(dtype, n):
+    """Four-step power-of-two lengths on the specialised kernels (2^20 in
+    complex128: interleaved column tiles of 1024 with the split exchange)."""
     rs = np.random.RandomState(11)
-    x = crandn(rs, (2, 1 << 18), dtype)
+    x = crandn(rs, (2, n), dtype)
     check("c2c", x, x, np.array([1]), True, 1.0)
-    check("c2c", x, x, np.array([1]), False, 1.0 / (1 << 18))
+    check("c2c", x, x, np.array([1]), False, 1.0 / n)
 
 
 @pytest.mark.parametrize("kind,type,n", [("dct", 2, 4099), ("dst", 3, 4099), ("dct", 4, 8192),
