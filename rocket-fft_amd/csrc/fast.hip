@@ -72,6 +72,23 @@ struct FCtx {
   int i;           // index of the thread within its line
 };
 
+// c2r: input p of the half-length complex transform from bins p and h - p
+// of the half spectrum
+template <typename T>
+__device__ __forceinline__ C2<T> c2r_pre(const LineArgs<T> &a, C2<T> xj, C2<T> xm, int p) {
+  if (a.sub & SUB_CONJ) {
+    xj.y = -xj.y;
+    xm.y = -xm.y;
+  }
+  if (p == 0) {
+    xj.y = T(0);
+    xm.y = T(0);
+  }
+  const C2<T> A = xj + conjc(xm);
+  const C2<T> B = cmul(xj - conjc(xm), conjc(a.t0[p]));
+  return C2<T>{A.x - B.y, A.y + B.x};
+}
+
 template <typename T, int OP, bool VIN>
 __device__ __forceinline__ C2<T> fload(const FCtx<T> &c, int p) {
   const LineArgs<T> &a = *c.a;
@@ -82,18 +99,7 @@ __device__ __forceinline__ C2<T> fload(const FCtx<T> &c, int p) {
       return C2<T>{c.gin[2 * p * a.esi], c.gin[(2 * p + 1) * a.esi]};
   } else if constexpr (OP == OP_C2R_EVEN) {
     const int h = a.f.n;
-    C2<T> xj = ldc<VIN>(c.gin + p * a.esi), xm = ldc<VIN>(c.gin + (h - p) * a.esi);
-    if (a.sub & SUB_CONJ) {
-      xj.y = -xj.y;
-      xm.y = -xm.y;
-    }
-    if (p == 0) {
-      xj.y = T(0);
-      xm.y = T(0);
-    }
-    const C2<T> A = xj + conjc(xm);
-    const C2<T> B = cmul(xj - conjc(xm), conjc(a.t0[p]));
-    return C2<T>{A.x - B.y, A.y + B.x};
+    return c2r_pre(a, ldc<VIN>(c.gin + p * a.esi), ldc<VIN>(c.gin + (h - p) * a.esi), p);
   } else if constexpr (OP == OP_PASSA) {
     if (a.sub & SUB_CHIRP) {  // Bluestein columns: x[m] times the chirp, zero from m = n on
       const int64_t m = int64_t(p) * a.other + c.aux;
@@ -181,7 +187,7 @@ __device__ __forceinline__ C2<T> r2c_bin(const LineArgs<T> &a, const C2<T> *line
 }
 
 template <typename T, int N, int TPL, int OP, bool VIN, bool VOUT, bool DIRECT_OUT,
-          typename SC, int S>
+          typename SC, int S, bool FLAT>
 __device__ __forceinline__ void fstage(const FCtx<T> &c) {
   constexpr int R = SC::at(S);
   constexpr int NS = SC::ns(S);
@@ -198,8 +204,11 @@ __device__ __forceinline__ void fstage(const FCtx<T> &c) {
     if (Q * TPL == NR || j < NR) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        if constexpr (FIRST)
+        if constexpr (FIRST && !FLAT)
           v[q][r] = c.valid ? fload<T, OP, VIN>(c, j + r * NR) : C2<T>{T(0), T(0)};
+        else if constexpr (FIRST && OP == OP_C2R_EVEN)
+          v[q][r] = c2r_pre(a, c.line[padi<T>(j + r * NR)], c.line[padi<T>(N - j - r * NR)],
+                            j + r * NR);
         else
           v[q][r] = c.line[padi<T>(j + r * NR)];
       }
@@ -226,7 +235,7 @@ __device__ __forceinline__ void fstage(const FCtx<T> &c) {
     }
     return;
   } else {
-    if constexpr (!FIRST) __syncthreads();  // every read of this stage is done
+    if constexpr (!FIRST || FLAT) __syncthreads();  // every read of this stage is done
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = c.i + q * TPL;
@@ -238,7 +247,7 @@ __device__ __forceinline__ void fstage(const FCtx<T> &c) {
     }
     __syncthreads();
     if constexpr (!LAST)
-      fstage<T, N, TPL, OP, VIN, VOUT, DIRECT_OUT, SC, S + 1>(c);
+      fstage<T, N, TPL, OP, VIN, VOUT, DIRECT_OUT, SC, S + 1, FLAT>(c);
   }
 }
 
@@ -380,10 +389,19 @@ template <typename T, int N, int TPL, int LPB, bool LF_IN, bool LF_OUT, int OP, 
 __global__ void __launch_bounds__(TPL *LPB, SPLIT ? 4 : 1) k_fast(const LineArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int P = pitch_of<T, N>();
-  constexpr bool DIRECT_OUT = (LF_IN == LF_OUT) && OP != OP_R2C_EVEN;
+  // flat mapping for lines of at most 16 (one thread per line): the tile is
+  // loaded and stored by consecutive lanes over consecutive elements (one
+  // contiguous run per wave instead of one element per lane on lines 128 B
+  // apart) through LDS.  Measured 1.6x on complex64 lines of 16; for longer
+  // lines (8+ lanes per line) the extra LDS pass and index division cost
+  // more than they save.
+  constexpr bool FLAT = !LF_IN && !LF_OUT && TPL == 1 && VIN && VOUT && !SPLIT &&
+                        (OP == OP_C2C || OP == OP_R2C_EVEN || OP == OP_C2R_EVEN);
+  constexpr bool DIRECT_OUT = (LF_IN == LF_OUT) && OP != OP_R2C_EVEN && !FLAT;
   C2<T> *lds = reinterpret_cast<C2<T> *>(smem_raw);
   T *slds = reinterpret_cast<T *>(smem_raw);
   __shared__ int64_t s_oo[LPB];
+  __shared__ int64_t s_oi[FLAT ? LPB : 1];
   __shared__ int s_aux[LPB];
   const int t = threadIdx.x;
   const int l = LF_IN ? t % LPB : t / TPL;
@@ -429,6 +447,18 @@ __global__ void __launch_bounds__(TPL *LPB, SPLIT ? 4 : 1) k_fast(const LineArgs
       s_aux[l] = aux;
     }
   }
+  if constexpr (FLAT) {
+    if (i == 0) s_oi[l] = oi;
+    __syncthreads();
+    const int nl = int(min(int64_t(LPB), a.nlines - tile * LPB));
+    constexpr int NIN = OP == OP_C2R_EVEN ? N + 1 : N;  // complex values per line
+    for (int f = t; f < nl * NIN; f += TPL * LPB) {
+      const int l2 = f / NIN, p = f - l2 * NIN;
+      const T *g = a.in + s_oi[l2];
+      lds[l2 * P + padi<T>(p)] = ldc<true>(g + (OP == OP_R2C_EVEN ? 2 * p : p * a.esi));
+    }
+    __syncthreads();
+  }
   if constexpr (SPLIT) {
     c.nl = int(min(int64_t(LPB), a.nlines - tile * LPB));
     c.lo = LF_OUT ? t % LPB : t / TPL;
@@ -440,9 +470,22 @@ __global__ void __launch_bounds__(TPL *LPB, SPLIT ? 4 : 1) k_fast(const LineArgs
     C2<T> v0[S0::Q][S0::R];
     fstage_split<T, N, TPL, OP, VIN, VOUT, DIRECT_OUT, SC, 0>(c, v0);
   } else {
-    fstage<T, N, TPL, OP, VIN, VOUT, DIRECT_OUT, SC, 0>(c);
+    fstage<T, N, TPL, OP, VIN, VOUT, DIRECT_OUT, SC, 0, FLAT>(c);
   }
-  if constexpr (!DIRECT_OUT && !SPLIT) {
+  if constexpr (FLAT) {
+    // natural-order result of every line is in LDS (the last stage ended
+    // with a barrier): consecutive lanes store consecutive outputs
+    const int nl = int(min(int64_t(LPB), a.nlines - tile * LPB));
+    constexpr int NOUT = OP == OP_R2C_EVEN ? N + 1 : N;
+    for (int f = t; f < nl * NOUT; f += TPL * LPB) {
+      const int l2 = f / NOUT, k = f - l2 * NOUT;
+      T *g = a.out + s_oo[l2];
+      if constexpr (OP == OP_R2C_EVEN)
+        stc<true>(g + k * a.eso, r2c_bin(a, lds + l2 * P, k));
+      else
+        fstore<T, OP, true>(a, c.twl, g, s_aux[l2], k, lds[l2 * P + padi<T>(k)]);
+    }
+  } else if constexpr (!DIRECT_OUT && !SPLIT) {
     // natural-order result of every line is in LDS; store with the output
     // mapping (the last stage ended with a barrier)
     const int nl = int(min(int64_t(LPB), a.nlines - tile * LPB));
