@@ -153,6 +153,17 @@ int rfh_batch_execute_serial(void *handle, void *stream);
 int rfh_batch_info(void *handle, int *grid, int *lds_bytes, int *autotuned, double *tuned_ms);
 int rfh_batch_destroy(void *handle);
 
+/* ---- per-line scaling (FFTLog transforms) -----------------------------------
+ * In place on device memory: nlines lines of len elements, line starts
+ * `stride` elements apart.  mode 0: real lines, x[l][j] *= w[j] (w: len
+ * doubles); mode 1: complex lines, X[l][k] = conj(X[l][k]) * c[k] (w: len
+ * complex values as 2*len doubles).  w is host memory, rounded to the data's
+ * precision.  Completes before returning.  No reference counterpart: it is
+ * the spectrum and bias step of the fht / ifht overloads
+ * (rocket_fft/overloads.py:858-902, 1755-1809) on the GPU. */
+int rfh_line_scale(void *data, int64_t nlines, int64_t len, int64_t stride, const double *w,
+                   int mode, bool double_precision, void *stream);
+
 /* Plan facts of the reference engine, restated bit-exactly:
  * complex FFTPACK factor list (_pocketfft_hdronly.h:1750-1778), real FFTPACK
  * factor list (:2629-2652), and the FFTPACK-vs-Bluestein decision

@@ -341,6 +341,29 @@ def mfma_probe(a_lanes, b_lanes, bf16=False):
     return d
 
 
+def line_scale(t, nlines, length, stride, w, complex_lines, double):
+    """In-place per-line scaling of a torch CUDA tensor (rfh_line_scale):
+    real lines times w (complex_lines False), or complex lines X -> conj(X) w
+    (w complex, complex_lines True).  Strides in elements."""
+    L = _load()
+    L.rfh_line_scale.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                 ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_bool,
+                                 ctypes.c_void_p]
+    L.rfh_line_scale.restype = ctypes.c_int
+    import torch
+    if complex_lines:
+        wv = np.ascontiguousarray(np.asarray(w, dtype=np.complex128).view(np.float64))
+    else:
+        wv = np.ascontiguousarray(np.asarray(w, dtype=np.float64))
+    stream = torch.cuda.current_stream(t.device).cuda_stream
+    rc = L.rfh_line_scale(ctypes.c_void_p(t.data_ptr()), int(nlines), int(length), int(stride),
+                          wv.ctypes.data_as(ctypes.c_void_p), 1 if complex_lines else 0,
+                          bool(double), ctypes.c_void_p(stream))
+    if rc:
+        raise RuntimeError(f"rfh_line_scale: {last_error()[1]}")
+    return t
+
+
 def set_fast_path(enabled):
     """Process-wide switch of the specialised power-of-two kernels (tests)."""
     _load().rfh_set_fast_path(bool(enabled))

@@ -23,15 +23,17 @@ Arrays may be NumPy arrays (host memory, staged through the GPU) or torch
 CUDA tensors (computed in HBM); results are of the same kind.
 """
 import numbers
+import warnings
 
 import numpy as np
 
-from . import c2c, c2c_sym, c2r, dct as _dct_ll, dst as _dst_ll, good_size, r2c
+from . import c2c, c2c_sym, c2r, dct as _dct_ll, dst as _dst_ll, good_size, line_scale, r2c
 
 __all__ = [
     "fft", "ifft", "fft2", "ifft2", "fftn", "ifftn", "rfft", "irfft", "rfft2", "irfft2",
     "rfftn", "irfftn", "hfft", "ihfft", "fftfreq", "rfftfreq", "fftshift", "ifftshift",
     "dct", "idct", "dst", "idst", "dctn", "idctn", "dstn", "idstn", "next_fast_len",
+    "fht", "ifht", "fhtoffset",
 ]
 
 
@@ -386,3 +388,114 @@ def idstn(x, type=2, s=None, axes=None, norm=None, overwrite_x=False, workers=No
 def next_fast_len(target, real=False):
     """scipy.fft.next_fast_len through good_size (overloads.py:1861-1870)."""
     return good_size(target, real)
+
+
+# --------------------------------------------------------- FFTLog (fht)
+
+def _loggamma_pair(xp, xm, y):
+    from scipy.special import loggamma
+    return loggamma(xp + 1j * y), loggamma(xm + 1j * y)
+
+
+def _fht_coeff(n, dln, mu, offset, bias):
+    """FFTLog coefficients u_m, m = 0..n/2 (overloads.py:858-883), from
+    u_m = 2^q (2 / k0r0)^{2 i y_m} Gamma(xp + i y_m) / Gamma(xm - i y_m) with
+    xp, xm = (mu + 1 +- q) / 2, y_m = pi m / (n dln), ln(k0r0) = offset,
+    q = bias; Gamma(xm - iy) = conj(Gamma(xm + iy)) gives the real part
+    Re lnG(xp+iy) - Re lnG(xm+iy) + q ln2 and the phase
+    Im lnG(xp+iy) + Im lnG(xm+iy) + 2 y (ln2 - offset)."""
+    xp = (mu + 1 + bias) / 2
+    xm = (mu + 1 - bias) / 2
+    y = np.linspace(0, np.pi * (n // 2) / (n * dln), n // 2 + 1)
+    gp, gm = _loggamma_pair(xp, xm, y)
+    ln2 = np.log(2)
+    u = np.exp((gp.real - gm.real + ln2 * bias) + 1j * (gp.imag + gm.imag + 2 * y * (ln2 - offset)))
+    u.imag[-1] = 0  # the last coefficient is real (overloads.py:877)
+    if not np.isfinite(u[0]):
+        # Gamma(xp) / Gamma(xm) with both at poles: its finite limit
+        from scipy.special import poch
+        u[0] = 2 ** bias * poch(xm, xp - xm)
+    return u
+
+
+def _fht_real_dtype(a):
+    if _is_torch(a):
+        if a.is_complex():
+            raise TypeError("The 1st argument must be a real array.")
+        return "float32" if str(a.dtype) == "torch.float32" else "float64"
+    if np.iscomplexobj(a):
+        raise TypeError("The 1st argument must be a real array.")
+    return "float32" if a.dtype == np.float32 else "float64"
+
+
+def _fht(a, dln, mu, offset, bias, inverse):
+    """fht / ifht on the GPU (overloads.py:1755-1809): bias weights, r2c,
+    the coefficient step and the reversal in one pass (rfh_line_scale), c2r,
+    output weights.  Host arrays are copied to the current CUDA device and
+    back."""
+    import torch
+    a = _asarray(a)
+    rt = _fht_real_dtype(a)
+    host = not _is_torch(a)
+    cast = np.float32 if rt == "float32" else np.float64
+    # the scalars are cast to the array's precision, as in the reference
+    dln, mu, offset, bias = (float(cast(v)) for v in (dln, mu, offset, bias))
+    t = torch.as_tensor(np.asarray(a, dtype=rt)) if host else a.to(getattr(torch, rt))
+    x = t.to("cuda" if host else t.device).contiguous().clone()
+    n = x.shape[-1]
+    dbl = rt == "float64"
+    lines = x.numel() // n if n else 0
+    j = np.arange(n)
+    jc = (n - 1) / 2
+    if bias != 0:
+        w_in = np.exp(bias * ((j - jc) * dln + offset)) if inverse else np.exp(-bias * (j - jc) * dln)
+        line_scale(x, lines, n, n, w_in, False, dbl)
+    u = _fht_coeff(n, dln, mu, offset, bias)
+    h = n // 2 + 1
+    ph = np.exp(2j * np.pi * np.arange(h) / n)
+    if not inverse:
+        if np.isinf(u[0]):
+            warnings.warn("singular transform; consider changing the bias", RuntimeWarning)
+            u = u.copy()
+            u[0] = 0
+        c = np.conj(u) * ph
+    else:
+        if u[0] == 0:
+            warnings.warn("singular inverse transform; consider changing the bias",
+                          RuntimeWarning)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            c = np.where(u == 0, 0, ph / np.where(u == 0, 1, u))
+    ct = "complex128" if dbl else "complex64"
+    X = torch.empty(tuple(x.shape[:-1]) + (h,), dtype=getattr(torch, ct), device=x.device)
+    ax = np.array([x.ndim - 1])
+    r2c(x, X, ax, True, 1.0)
+    line_scale(X, lines, h, h, c, True, dbl)
+    out = torch.empty_like(x)
+    c2r(X, out, ax, False, 1.0 / n)
+    if bias != 0:
+        w_out = (1 / np.exp(-bias * (j - jc) * dln) if inverse
+                 else np.exp(-bias * ((j - jc) * dln + offset)))
+        line_scale(out, lines, n, n, w_out, False, dbl)
+    return out.cpu().numpy() if host else out
+
+
+def fht(a, dln, mu, offset=0.0, bias=0.0):
+    """scipy.fft.fht: fast Hankel transform (FFTLog) over the last axis."""
+    return _fht(a, dln, mu, offset, bias, False)
+
+
+def ifht(A, dln, mu, offset=0.0, bias=0.0):
+    """scipy.fft.ifht: inverse fast Hankel transform over the last axis."""
+    return _fht(A, dln, mu, offset, bias, True)
+
+
+def fhtoffset(dln, mu, initial=0.0, bias=0.0):
+    """scipy.fft.fhtoffset: the low-ringing offset nearest to ``initial``
+    (overloads.py:1845-1858): ln(kr) moved by the fraction of dln that makes
+    the phase of the coefficient at the Nyquist frequency a multiple of pi."""
+    xp = (mu + 1 + bias) / 2
+    xm = (mu + 1 - bias) / 2
+    y = np.pi / (2 * dln)
+    gp, gm = _loggamma_pair(xp, xm, y)
+    arg = (np.log(2) - initial) / dln + (gp.imag + gm.imag) / np.pi
+    return initial + (arg - np.round(arg)) * dln
