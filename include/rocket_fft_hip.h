@@ -164,6 +164,15 @@ int rfh_batch_destroy(void *handle);
 int rfh_line_scale(void *data, int64_t nlines, int64_t len, int64_t stride, const double *w,
                    int mode, bool double_precision, void *stream);
 
+/* Cyclic shift, numpy.roll over every axis at once: out (C-contiguous, shape
+ * `shape`) [i] = in[(i_k - shifts[k]) mod shape[k]], input strides in
+ * elements (any sign), elements of 1, 2, 4, 8 or 16 bytes, device memory,
+ * enqueued on `stream` (out must not overlap in).  No reference counterpart
+ * in the C ABI: it is the roll / fftshift / ifftshift overloads
+ * (rocket_fft/overloads.py:762-855, 1253-1310) on the GPU. */
+int rfh_roll(const void *in, void *out, int ndim, const int64_t *shape, const int64_t *in_strides,
+             const int64_t *shifts, int64_t elem_bytes, void *stream);
+
 /* Plan facts of the reference engine, restated bit-exactly:
  * complex FFTPACK factor list (_pocketfft_hdronly.h:1750-1778), real FFTPACK
  * factor list (:2629-2652), and the FFTPACK-vs-Bluestein decision

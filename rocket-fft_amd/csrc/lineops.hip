@@ -1,5 +1,6 @@
-// lineops.hip -- elementwise per-line scaling used by the FFTLog (fast
-// Hankel) transforms of the front-end.
+// lineops.hip -- data-movement and elementwise operations beside the
+// transforms: per-line scaling for the FFTLog (fast Hankel) transforms, and
+// the cyclic shift behind roll / fftshift / ifftshift.
 //
 // The FFTLog transform of a real line a (rocket_fft/overloads.py:858-902,
 // 1755-1809) is rfft(a * w_in) * u, back through irfft, reversed, times
@@ -60,8 +61,113 @@ void line_scale(void *data, int64_t nlines, int64_t len, int64_t stride, const d
   HIP_CHECK(hipStreamSynchronize(s));  // dw is freed on return
 }
 
+// roll: out (C-contiguous) [i_0, ..., i_{d-1}] = in[(i_k - sh_k) mod n_k] with
+// arbitrary input strides (elements); one element per thread, consecutive
+// threads on consecutive outputs.
+constexpr int kRollDims = 32;
+struct RollGeom {
+  int ndim;
+  int64_t shape[kRollDims];
+  int64_t istr[kRollDims];
+  int64_t shift[kRollDims];  // in [0, n_k)
+};
+
+template <typename E>
+__global__ void __launch_bounds__(256) k_roll(const E *in, E *out, int64_t total,
+                                              const RollGeom g) {
+  for (int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; f < total;
+       f += int64_t(gridDim.x) * blockDim.x) {
+    int64_t rem = f, src = 0;
+    for (int d = g.ndim - 1; d >= 0; --d) {
+      const int64_t n = g.shape[d];
+      const int64_t q = rem / n;
+      int64_t i = rem - q * n - g.shift[d];
+      rem = q;
+      if (i < 0) i += n;
+      src += i * g.istr[d];
+    }
+    out[f] = in[src];
+  }
+}
+
+struct alignas(16) E16 {
+  uint64_t a, b;
+};
+
+template <typename E>
+void roll_launch(const void *in, void *out, int64_t total, const RollGeom &g, hipStream_t s) {
+  const unsigned grid = unsigned(std::min<int64_t>((total + 255) / 256, 16384));
+  hipLaunchKernelGGL((k_roll<E>), dim3(grid), dim3(256), 0, s, static_cast<const E *>(in),
+                     static_cast<E *>(out), total, g);
+  HIP_CHECK(hipGetLastError());
+}
+
+bool on_device(const void *p, int *dev) {
+  hipPointerAttribute_t at{};
+  if (hipPointerGetAttributes(&at, p) != hipSuccess ||
+      (at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged)) {
+    (void)hipGetLastError();
+    return false;
+  }
+  *dev = at.device;
+  return true;
+}
+
 }  // namespace
 }  // namespace rfh
+
+extern "C" int rfh_roll(const void *in, void *out, int ndim, const int64_t *shape,
+                        const int64_t *in_strides, const int64_t *shifts, int64_t elem_bytes,
+                        void *stream) {
+  using namespace rfh;
+  try {
+    if (ndim < 0 || ndim > kRollDims) RFH_THROW(1, "rfh_roll: ndim out of range");
+    RollGeom g{};
+    g.ndim = ndim;
+    int64_t total = 1;
+    for (int d = 0; d < ndim; ++d) {
+      if (shape[d] < 0) RFH_THROW(1, "rfh_roll: negative extent");
+      g.shape[d] = shape[d];
+      g.istr[d] = in_strides[d];
+      total *= shape[d];
+    }
+    if (total == 0) return 0;
+    for (int d = 0; d < ndim; ++d) {
+      const int64_t n = shape[d];
+      int64_t sh = shifts[d] % n;
+      if (sh < 0) sh += n;
+      g.shift[d] = sh;
+    }
+    int dev_in = 0, dev_out = 0;
+    if (!in || !out || !on_device(in, &dev_in) || !on_device(out, &dev_out) || dev_in != dev_out)
+      RFH_THROW(1, "rfh_roll: in and out must be device memory of one device");
+    int prev = 0;
+    HIP_CHECK(hipGetDevice(&prev));
+    if (prev != dev_out) HIP_CHECK(hipSetDevice(dev_out));
+    struct Restore {
+      int d, cur;
+      ~Restore() {
+        if (d != cur) (void)hipSetDevice(d);
+      }
+    } rs{prev, dev_out};
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    switch (elem_bytes) {
+      case 1: roll_launch<uint8_t>(in, out, total, g, s); break;
+      case 2: roll_launch<uint16_t>(in, out, total, g, s); break;
+      case 4: roll_launch<uint32_t>(in, out, total, g, s); break;
+      case 8: roll_launch<uint64_t>(in, out, total, g, s); break;
+      case 16: roll_launch<E16>(in, out, total, g, s); break;
+      default: RFH_THROW(1, "rfh_roll: element size must be 1, 2, 4, 8 or 16 bytes");
+    }
+    return 0;
+  } catch (const Error &e) {
+    set_error(e.code, e.msg);
+    return e.code;
+  } catch (const std::exception &e) {
+    set_error(1, e.what());
+    return 1;
+  }
+}
 
 extern "C" int rfh_line_scale(void *data, int64_t nlines, int64_t len, int64_t stride,
                               const double *w, int mode, bool double_precision, void *stream) {

@@ -364,6 +364,30 @@ def line_scale(t, nlines, length, stride, w, complex_lines, double):
     return t
 
 
+def roll_device(t, out, shifts):
+    """out[i] = t[(i - shifts) mod shape] over every axis (rfh_roll): t a
+    torch CUDA tensor of any strides, out a C-contiguous tensor of the same
+    shape and dtype on the same device, enqueued on the current stream."""
+    L = _load()
+    L.rfh_roll.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+    L.rfh_roll.restype = ctypes.c_int
+    import torch
+    if tuple(out.shape) != tuple(t.shape) or out.dtype != t.dtype or not out.is_contiguous():
+        raise ValueError("roll_device: out must be a contiguous tensor like t")
+    nd = t.dim()
+    shape = np.array(t.shape, dtype=np.int64)
+    strides = np.array(t.stride(), dtype=np.int64)
+    sh = np.array(shifts, dtype=np.int64).reshape(nd)
+    stream = torch.cuda.current_stream(t.device).cuda_stream
+    rc = L.rfh_roll(ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(out.data_ptr()), nd,
+                    shape.ctypes.data_as(ctypes.c_void_p), strides.ctypes.data_as(ctypes.c_void_p),
+                    sh.ctypes.data_as(ctypes.c_void_p), t.element_size(), ctypes.c_void_p(stream))
+    if rc:
+        raise RuntimeError(f"rfh_roll: {last_error()[1]}")
+    return out
+
+
 def set_fast_path(enabled):
     """Process-wide switch of the specialised power-of-two kernels (tests)."""
     _load().rfh_set_fast_path(bool(enabled))

@@ -27,13 +27,14 @@ import warnings
 
 import numpy as np
 
-from . import c2c, c2c_sym, c2r, dct as _dct_ll, dst as _dst_ll, good_size, line_scale, r2c
+from . import (c2c, c2c_sym, c2r, dct as _dct_ll, dst as _dst_ll, good_size, line_scale, r2c,
+               roll_device)
 
 __all__ = [
     "fft", "ifft", "fft2", "ifft2", "fftn", "ifftn", "rfft", "irfft", "rfft2", "irfft2",
     "rfftn", "irfftn", "hfft", "ihfft", "fftfreq", "rfftfreq", "fftshift", "ifftshift",
     "dct", "idct", "dst", "idst", "dctn", "idctn", "dstn", "idstn", "next_fast_len",
-    "fht", "ifht", "fhtoffset",
+    "fht", "ifht", "fhtoffset", "roll",
 ]
 
 
@@ -287,16 +288,58 @@ def rfftfreq(n, d=1.0, device=None):
     return np.arange(n // 2 + 1) * val
 
 
+def _roll_gpu(x, shifts):
+    """Cyclic shift of every axis by shifts[k] in one gather kernel (rfh_roll).
+    Host arrays are copied to the current CUDA device and back."""
+    import torch
+    host = not _is_torch(x)
+    if host:
+        try:
+            t = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+        except TypeError:
+            raise TypeError(f"roll: unsupported dtype {x.dtype}") from None
+    else:
+        t = x
+    out = torch.empty(tuple(t.shape), dtype=t.dtype, device=t.device)
+    if t.numel():
+        roll_device(t, out, shifts)
+    return out.cpu().numpy() if host else out
+
+
+def roll(a, shift, axis=None):
+    """numpy.roll (overloads.py:762-855): with ``axis`` None the flattened
+    array is shifted by the sum of ``shift`` (a scalar or 1-D sequence);
+    otherwise ``axis`` and ``shift`` broadcast against each other, shifts
+    on one axis add up, and every axis moves in the same gather kernel."""
+    if np.isscalar(a):
+        return np.asarray(a)
+    x = _asarray(a)
+    if axis is None:
+        sh = np.asarray(shift)
+        if sh.ndim > 1:
+            raise ValueError("'shift' must be a scalar or 1D sequence")
+        flat = x.reshape(-1) if _is_torch(x) else np.ravel(x)
+        return _roll_gpu(flat, [int(sh.sum())]).reshape(tuple(x.shape))
+    ax, sh = np.broadcast_arrays(np.asarray(axis), np.asarray(shift))
+    shifts = [0] * x.ndim
+    for k, v in zip(ax.ravel(), sh.ravel()):
+        k = int(k)
+        if k >= x.ndim or k < -x.ndim:
+            raise ValueError("axis is out of bounds")
+        shifts[k % x.ndim] += int(v)
+    return _roll_gpu(x, shifts)
+
+
 def _shift(x, axes, sign):
     if axes is None:
         axes = list(range(x.ndim))
     elif isinstance(axes, numbers.Integral):
         axes = [axes]
-    shifts = [sign * (x.shape[ax] // 2) for ax in axes]
-    if _is_torch(x):
-        import torch
-        return torch.roll(x, shifts, dims=list(axes))
-    return np.roll(x, shifts, axis=tuple(axes))
+    shifts = [0] * x.ndim
+    for ax in axes:
+        ax = _wrap(ax, x.ndim)
+        shifts[ax] += sign * (x.shape[ax] // 2)
+    return _roll_gpu(x, shifts)
 
 
 def fftshift(x, axes=None):
