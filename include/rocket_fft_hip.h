@@ -150,7 +150,16 @@ int rfh_batch_execute(void *handle, void *stream);
 /* The same jobs launched one transform at a time through the per-call
  * engine (the comparison point of the persistent launch). */
 int rfh_batch_execute_serial(void *handle, void *stream);
+/* autotuned: 0 = not tuned, 1 = timed by this call, 2 = result remembered by
+ * this process, 3 = read from the cache directory. */
 int rfh_batch_info(void *handle, int *grid, int *lds_bytes, int *autotuned, double *tuned_ms);
+/* Directory of the on-disk autotune cache (NULL or "" = none, the default):
+ * results are read from and appended to <dir>/ragged_tune_<arch>_<CUs>cu.txt,
+ * keyed by a 128-bit hash of the batch signature (precision, kind and length
+ * of every job).  clear_memory forgets this process's remembered results.
+ * The directory must exist; an unwritable one only disables writing.
+ * Process-wide. */
+int rfh_tune_cache(const char *dir, bool clear_memory);
 int rfh_batch_destroy(void *handle);
 
 /* ---- per-line scaling (FFTLog transforms) -----------------------------------

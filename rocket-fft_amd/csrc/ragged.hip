@@ -15,12 +15,21 @@
 //
 // Execution allocates nothing and calls no host synchronisation, so it can
 // be captured in a HIP graph.
+#include <fcntl.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <cctype>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <mutex>
 #include <set>
+#include <sstream>
+#include <string>
+#include <tuple>
 
 #include "../../include/rocket_fft_hip.h"
 #include "passes.hpp"
@@ -351,8 +360,23 @@ std::unique_ptr<Built> build_any(const Batch &bt, size_t budget) {
 
 // Autotuner: LDS budget (which sets the four-step splits and how many
 // workgroups share a CU) x grid size; each candidate is timed on the batch.
+// Results are kept per batch signature (precision, then kind and length of
+// every job) for the process and, when a cache directory is set, in a text
+// file per GPU architecture and CU count that later processes read:
+//   <dir>/ragged_tune_<arch>_<cus>cu.txt, one line per signature:
+//   "rfh1 <fnv-1a hash> <second hash> <jobs> <lds budget> <grid>"
+// Lines are appended with one write each; unreadable lines are ignored.
 std::mutex g_tune_mu;
 std::map<std::vector<uint64_t>, std::pair<size_t, int>> g_tuned;
+std::string g_cache_dir;  // empty: no disk cache
+struct DiskKey {
+  uint64_t h1, h2, n;
+  bool operator<(const DiskKey &o) const {
+    return std::tie(h1, h2, n) < std::tie(o.h1, o.h2, o.n);
+  }
+};
+std::map<DiskKey, std::pair<size_t, int>> g_disk;
+std::string g_disk_loaded;  // file whose lines are in g_disk
 
 std::vector<uint64_t> signature(const Batch &bt) {
   std::vector<uint64_t> sig{uint64_t(bt.dbl)};
@@ -360,8 +384,69 @@ std::vector<uint64_t> signature(const Batch &bt) {
   return sig;
 }
 
+DiskKey disk_key(const std::vector<uint64_t> &sig) {
+  uint64_t h1 = 1469598103934665603ull, h2 = 0x9e3779b97f4a7c15ull;
+  for (uint64_t v : sig) {
+    for (int b = 0; b < 8; ++b) {
+      h1 = (h1 ^ ((v >> (8 * b)) & 0xff)) * 1099511628211ull;
+    }
+    h2 ^= v + 0x9e3779b97f4a7c15ull + (h2 << 6) + (h2 >> 2);
+    h2 *= 0xff51afd7ed558ccdull;
+  }
+  return DiskKey{h1, h2, uint64_t(sig.size())};
+}
+
+bool budget_ok(size_t b) { return b == 32768 || b == 65536 || b == std::min<size_t>(98304, lds_limit() - 8192); }
+
+std::string cache_file() {
+  if (g_cache_dir.empty()) return {};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  hipDeviceProp_t pr;
+  std::string arch = "gpu";
+  if (hipGetDeviceProperties(&pr, dev) == hipSuccess) {
+    arch.clear();
+    for (const char *c = pr.gcnArchName; *c && *c != ':'; ++c)
+      arch += std::isalnum(static_cast<unsigned char>(*c)) ? *c : '_';
+  }
+  return g_cache_dir + "/ragged_tune_" + arch + "_" + std::to_string(cu_count()) + "cu.txt";
+}
+
+// with g_tune_mu held
+void load_disk(const std::string &path) {
+  if (path == g_disk_loaded) return;
+  g_disk.clear();
+  g_disk_loaded = path;
+  std::ifstream f(path);
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    std::string tag;
+    DiskKey k{};
+    size_t budget = 0;
+    int grid = 0;
+    if (!(is >> tag >> std::hex >> k.h1 >> k.h2 >> std::dec >> k.n >> budget >> grid)) continue;
+    if (tag != "rfh1" || !budget_ok(budget) || grid <= 0 || grid > 64 * cu_count()) continue;
+    g_disk[k] = {budget, grid};
+  }
+}
+
+void append_disk(const std::string &path, const DiskKey &k, size_t budget, int grid) {
+  char buf[160];
+  const int len = std::snprintf(buf, sizeof buf, "rfh1 %016llx %016llx %llu %zu %d\n",
+                                (unsigned long long)k.h1, (unsigned long long)k.h2,
+                                (unsigned long long)k.n, budget, grid);
+  const int fd = ::open(path.c_str(), O_WRONLY | O_CREAT | O_APPEND, 0644);
+  if (fd < 0) return;  // the cache is an optimisation: an unwritable directory is not an error
+  const ssize_t w = ::write(fd, buf, size_t(len));
+  (void)w;  // a short write leaves a line that later reads ignore
+  ::close(fd);
+}
+
 void autotune(Batch &bt, hipStream_t s) {
   const auto sig = signature(bt);
+  const DiskKey dk = disk_key(sig);
+  std::string path;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tuned.find(sig);
@@ -370,6 +455,26 @@ void autotune(Batch &bt, hipStream_t s) {
       bt.grid = it->second.second;
       bt.autotuned = 2;
       return;
+    }
+    path = cache_file();
+    if (!path.empty()) {
+      load_disk(path);
+      auto d = g_disk.find(dk);
+      if (d != g_disk.end()) {
+        std::unique_ptr<Built> b;
+        try {
+          b = build_any(bt, d->second.first);
+        } catch (const Error &) {
+          // a cached budget that no longer plans (another build): tune again
+        }
+        if (b) {
+          bt.b = std::move(b);
+          bt.grid = d->second.second;
+          bt.autotuned = 3;
+          g_tuned[sig] = d->second;
+          return;
+        }
+      }
     }
   }
   const size_t cap = lds_limit() - 8192;
@@ -404,6 +509,10 @@ void autotune(Batch &bt, hipStream_t s) {
   bt.tuned_ms = best;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned[sig] = {best_budget, best_grid};
+  if (!path.empty() && path == cache_file()) {
+    append_disk(path, dk, best_budget, best_grid);
+    g_disk[dk] = {best_budget, best_grid};
+  }
 }
 
 template <typename F>
@@ -513,6 +622,16 @@ int rfh_batch_execute_serial(void *handle, void *stream) {
   });
   rfh_set_stream(prev_s, prev_b);
   return rc;
+}
+
+int rfh_tune_cache(const char *dir, bool clear_memory) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (clear_memory) g_tuned.clear();
+  g_cache_dir = dir ? std::string(dir) : std::string();
+  while (g_cache_dir.size() > 1 && g_cache_dir.back() == '/') g_cache_dir.pop_back();
+  g_disk.clear();
+  g_disk_loaded.clear();
+  return 0;
 }
 
 int rfh_batch_info(void *handle, int *grid, int *lds_bytes, int *autotuned, double *tuned_ms) {

@@ -493,6 +493,17 @@ class RaggedBatch:
             pass
 
 
+def set_tune_cache(directory=None, clear_memory=False):
+    """Directory of the on-disk ragged-batch autotune cache (None: none),
+    read and appended by RaggedBatch(autotune=True); clear_memory forgets the
+    results remembered by this process (rfh_tune_cache)."""
+    L = _load()
+    L.rfh_tune_cache.argtypes = [ctypes.c_char_p, ctypes.c_bool]
+    L.rfh_tune_cache.restype = ctypes.c_int
+    d = None if directory is None else os.fsencode(str(directory))
+    L.rfh_tune_cache(d, bool(clear_memory))
+
+
 def ragged_workload(njobs=4096, seed=5, lo=64, hi=65536, good_frac=0.7, c2c_frac=0.6):
     """The ragged-batch workload: lengths log-uniform in [lo, hi] from
     RandomState(seed); good_frac of them rounded up with good_size (complex

@@ -126,3 +126,44 @@ def test_ragged_workload_sample():
                else torch.fft.rfft(x.to(torch.float64)))
         assert float(torch.linalg.norm(y.to(ref.dtype) - ref) / torch.linalg.norm(ref)) < 3e-6
     b.close()
+
+
+def test_ragged_tune_cache_on_disk(tmp_path):
+    """Autotune results persist in the cache directory: a process with an
+    empty memory cache reads them back (autotuned == 3) and runs the same
+    configuration with bit-identical output; unreadable lines are skipped;
+    a new signature is timed and appended."""
+    rs = np.random.RandomState(21)
+    jobs, _ = make_jobs(np.complex64, [64, 1000, 4099, 257, 8192, 12], rs)
+    R.set_tune_cache(tmp_path, clear_memory=True)
+    try:
+        b1 = R.RaggedBatch(jobs, autotune=True)
+        i1 = b1.info()
+        assert i1["autotuned"] == 1
+        files = list(tmp_path.iterdir())
+        assert len(files) == 1 and files[0].name.startswith("ragged_tune_gfx")
+        assert [l.split()[0] for l in files[0].read_text().splitlines()] == ["rfh1"]
+        b1.execute()
+        torch.cuda.synchronize()
+        first = [j[4].clone() for j in jobs]
+        with open(files[0], "a") as f:
+            f.write("garbage line\nrfh1 zz\n")
+        R.set_tune_cache(tmp_path, clear_memory=True)  # as a fresh process
+        b2 = R.RaggedBatch(jobs, autotune=True)
+        i2 = b2.info()
+        assert i2["autotuned"] == 3
+        assert (i2["grid"], i2["lds_bytes"]) == (i1["grid"], i1["lds_bytes"])
+        for j in jobs:
+            j[4].zero_()
+        b2.execute()
+        torch.cuda.synchronize()
+        for a, j in zip(first, jobs):
+            assert torch.equal(a, j[4])
+        assert R.RaggedBatch(jobs, autotune=True).info()["autotuned"] == 2
+        jobs2, _ = make_jobs(np.complex64, [64, 1000, 4099, 257, 8192, 13], rs)
+        assert R.RaggedBatch(jobs2, autotune=True).info()["autotuned"] == 1
+        good = [l for l in files[0].read_text().splitlines()
+                if l.startswith("rfh1 ") and len(l.split()) == 6]
+        assert len(good) == 2
+    finally:
+        R.set_tune_cache(None)
