@@ -44,9 +44,9 @@ BYTES_C2C_PASS = 2 * N3 * N3 * (N3 // 2 + 1) * 8
 #   rocprofv3 --pmc FETCH_SIZE -- python3 bench.py --probe   (then WRITE_SIZE)
 # each counter scaled by what it reads for the device copy of known size
 # (67,633,152 B each way) in the same run: FETCH_SIZE x 1.999, WRITE_SIZE
-# x 1.000 on MI355X.  Mean of six launches: 67,767,836 B fetched +
-# 68,723,963 B written = 1.009 x the algorithmic bytes.
-TRAFFIC_C2C_PASS = 136491799
+# x 1.000 on MI355X.  Mean of six launches: 67,794,945 B fetched +
+# 68,875,792 B written = 1.010 x the algorithmic bytes.
+TRAFFIC_C2C_PASS = 136670737
 
 
 def rel_l2(a, b):
@@ -60,8 +60,11 @@ def inputs():
     return x, d
 
 
-def graph_time(torch, fn, stream, reps):
-    """Capture fn once, replay `reps` times; mean ms per replay (events)."""
+def graph_time(torch, fn, stream, reps, batches=5):
+    """Capture fn once and replay it: `batches` timed batches of `reps`
+    replays each (events on `stream`); the median over batches of the mean
+    ms per replay, so that one slow batch does not move a short kernel's
+    figure."""
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g, stream=stream):
         fn()
@@ -69,13 +72,16 @@ def graph_time(torch, fn, stream, reps):
     g.replay()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    with torch.cuda.stream(stream):
-        e0.record(stream)
-        for _ in range(reps):
-            g.replay()
-        e1.record(stream)
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / reps, g
+    times = []
+    for _ in range(batches):
+        with torch.cuda.stream(stream):
+            e0.record(stream)
+            for _ in range(reps):
+                g.replay()
+            e1.record(stream)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) / reps)
+    return float(np.median(times)), g
 
 
 def cpu_baseline(x, d, seconds):
@@ -403,7 +409,7 @@ def main():
                      "traffic_over_algorithmic": round(TRAFFIC_C2C_PASS / BYTES_C2C_PASS, 3)},
     }
     if not args.no_extra:
-        res["configs"] = extra_configs(torch, R, stream, 10)
+        res["configs"] = extra_configs(torch, R, stream, 20)
         res["configs"]["ragged_4096_c64_r2c"] = ragged_config(torch, R, stream)
         res["configs"]["precision_c64_16384x4096"] = precision_config(torch, R, stream)
         res["configs"]["lengths_batch256"] = lengths_config(torch, R, stream, 50)
