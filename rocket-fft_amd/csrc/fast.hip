@@ -59,6 +59,7 @@ template <typename T>
 struct FCtx {
   const LineArgs<T> *a;
   const C2<T> *twl; // four-step twiddle tables in LDS (pass A), or null
+  const C2<T> *rtw; // real-transform twiddles t0 (an LDS copy for short lines)
   C2<T> *line;     // this thread's line in LDS
   T *sline;        // ... in the split layout (one part at a time)
   T *soline;       // line of the output mapping in the split layout
@@ -75,7 +76,8 @@ struct FCtx {
 // c2r: input p of the half-length complex transform from bins p and h - p
 // of the half spectrum
 template <typename T>
-__device__ __forceinline__ C2<T> c2r_pre(const LineArgs<T> &a, C2<T> xj, C2<T> xm, int p) {
+__device__ __forceinline__ C2<T> c2r_pre(const LineArgs<T> &a, const C2<T> *t0, C2<T> xj,
+                                         C2<T> xm, int p) {
   if (a.sub & SUB_CONJ) {
     xj.y = -xj.y;
     xm.y = -xm.y;
@@ -85,7 +87,7 @@ __device__ __forceinline__ C2<T> c2r_pre(const LineArgs<T> &a, C2<T> xj, C2<T> x
     xm.y = T(0);
   }
   const C2<T> A = xj + conjc(xm);
-  const C2<T> B = cmul(xj - conjc(xm), conjc(a.t0[p]));
+  const C2<T> B = cmul(xj - conjc(xm), conjc(t0[p]));
   return C2<T>{A.x - B.y, A.y + B.x};
 }
 
@@ -99,7 +101,7 @@ __device__ __forceinline__ C2<T> fload(const FCtx<T> &c, int p) {
       return C2<T>{c.gin[2 * p * a.esi], c.gin[(2 * p + 1) * a.esi]};
   } else if constexpr (OP == OP_C2R_EVEN) {
     const int h = a.f.n;
-    return c2r_pre(a, ldc<VIN>(c.gin + p * a.esi), ldc<VIN>(c.gin + (h - p) * a.esi), p);
+    return c2r_pre(a, c.rtw, ldc<VIN>(c.gin + p * a.esi), ldc<VIN>(c.gin + (h - p) * a.esi), p);
   } else if constexpr (OP == OP_PASSA) {
     if (a.sub & SUB_CHIRP) {  // Bluestein columns: x[m] times the chirp, zero from m = n on
       const int64_t m = int64_t(p) * a.other + c.aux;
@@ -169,7 +171,8 @@ __device__ __forceinline__ void fstore(const LineArgs<T> &a, const C2<T> *twl, T
 // r2c post-processing of bin k (0..h) from the natural-order half-length
 // spectrum in LDS
 template <typename T>
-__device__ __forceinline__ C2<T> r2c_bin(const LineArgs<T> &a, const C2<T> *line, int k) {
+__device__ __forceinline__ C2<T> r2c_bin(const LineArgs<T> &a, const C2<T> *t0, const C2<T> *line,
+                                         int k) {
   const int h = a.f.n;
   C2<T> X;
   if (k == 0 || k == h) {
@@ -180,14 +183,39 @@ __device__ __forceinline__ C2<T> r2c_bin(const LineArgs<T> &a, const C2<T> *line
     const C2<T> fe = T(0.5) * (zk + zm);
     const C2<T> dd = zk - zm;
     const C2<T> fo = C2<T>{T(0.5) * dd.y, T(-0.5) * dd.x};
-    X = fe + cmul(a.t0[k], fo);
+    X = fe + cmul(t0[k], fo);
   }
   if (a.sub & SUB_CONJ) X.y = -X.y;
   return a.fct * X;
 }
 
+// c2r stage 0 with the bins h - p taken from the lanes that loaded them:
+// with one butterfly per thread (p = j + r TPL, TPL lanes of a line adjacent
+// in the wave), bin N - p is bin (TPL - j) + (R - 1 - r) TPL of lane
+// TPL - j, or, for j = 0, bin (R - r) TPL of the lane itself (bin N loaded
+// separately); each input is read from memory once instead of twice.
+template <typename T, int N, int TPL, int R, bool VIN>
+__device__ __forceinline__ void c2r_load_shuffled(const FCtx<T> &c, C2<T> (&v)[R]) {
+  const LineArgs<T> &a = *c.a;
+  const int j = c.i;
+  C2<T> x[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    x[r] = c.valid ? ldc<VIN>(c.gin + (j + r * TPL) * a.esi) : C2<T>{T(0), T(0)};
+  const C2<T> top = (c.valid && j == 0) ? ldc<VIN>(c.gin + N * a.esi) : C2<T>{T(0), T(0)};
+  const int lane = threadIdx.x & 63;
+  const int src = j == 0 ? lane : lane + TPL - 2 * j;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const C2<T> w = x[R - 1 - r];
+    C2<T> m{__shfl(w.x, src), __shfl(w.y, src)};
+    if (j == 0) m = r == 0 ? top : x[(R - r) % R];
+    v[r] = c2r_pre(a, c.rtw, x[r], m, j + r * TPL);
+  }
+}
+
 template <typename T, int N, int TPL, int OP, bool VIN, bool VOUT, bool DIRECT_OUT,
-          typename SC, int S, bool FLAT>
+          typename SC, int S, bool FLAT, bool SHUF = false>
 __device__ __forceinline__ void fstage(const FCtx<T> &c) {
   constexpr int R = SC::at(S);
   constexpr int NS = SC::ns(S);
@@ -198,16 +226,20 @@ __device__ __forceinline__ void fstage(const FCtx<T> &c) {
   const LineArgs<T> &a = *c.a;
   const T dir = a.dir;
   C2<T> v[Q][R];
+  static_assert(!SHUF || (FIRST && Q == 1 && NR == TPL && OP == OP_C2R_EVEN), "shuffled c2r load");
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int j = c.i + q * TPL;
     if (Q * TPL == NR || j < NR) {
+      if constexpr (SHUF) c2r_load_shuffled<T, N, TPL, R, VIN>(c, v[q]);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        if constexpr (FIRST && !FLAT)
+        if constexpr (SHUF)
+          ;
+        else if constexpr (FIRST && !FLAT)
           v[q][r] = c.valid ? fload<T, OP, VIN>(c, j + r * NR) : C2<T>{T(0), T(0)};
         else if constexpr (FIRST && OP == OP_C2R_EVEN)
-          v[q][r] = c2r_pre(a, c.line[padi<T>(j + r * NR)], c.line[padi<T>(N - j - r * NR)],
+          v[q][r] = c2r_pre(a, c.rtw, c.line[padi<T>(j + r * NR)], c.line[padi<T>(N - j - r * NR)],
                             j + r * NR);
         else
           v[q][r] = c.line[padi<T>(j + r * NR)];
@@ -376,8 +408,16 @@ __device__ __forceinline__ void fstage_split(
   }
 }
 
+// r2c / c2r of half-length N <= 512 keep their N post/pre twiddles in LDS
+// behind the lines (read once per bin; from global memory they cost one
+// more vector memory instruction per element)
+template <typename T, int N, int OP>
+constexpr bool rtw_in_lds() {
+  return (OP == OP_R2C_EVEN || OP == OP_C2R_EVEN) && N <= 512;
+}
+
 // LDS bytes of the lines of a workgroup (the four-step twiddle tables of
-// pass A, when copied, follow at this offset)
+// pass A, when copied, or the real-transform twiddles follow at this offset)
 template <typename T, int N, int LPB, bool SPLIT>
 constexpr size_t lines_bytes() {
   return SPLIT ? (size_t(LPB) * pitch_of<T, N>() * sizeof(T) + 15) / 16 * 16
@@ -420,6 +460,13 @@ __global__ void __launch_bounds__(TPL *LPB, SPLIT ? 4 : 1) k_fast(const LineArgs
       __syncthreads();
       c.twl = twl;
     }
+  }
+  c.rtw = a.t0;
+  if constexpr (rtw_in_lds<T, N, OP>()) {
+    C2<T> *rtw = reinterpret_cast<C2<T> *>(smem_raw + lines_bytes<T, N, LPB, SPLIT>());
+    for (int q = t; q < N; q += TPL * LPB) rtw[q] = a.t0[q];
+    __syncthreads();
+    c.rtw = rtw;
   }
   c.line = lds + l * P;
   c.sline = slds + l * P;
@@ -470,7 +517,11 @@ __global__ void __launch_bounds__(TPL *LPB, SPLIT ? 4 : 1) k_fast(const LineArgs
     C2<T> v0[S0::Q][S0::R];
     fstage_split<T, N, TPL, OP, VIN, VOUT, DIRECT_OUT, SC, 0>(c, v0);
   } else {
-    fstage<T, N, TPL, OP, VIN, VOUT, DIRECT_OUT, SC, 0, FLAT>(c);
+    // c2r on contiguous lines with one stage-0 butterfly per thread: mirrored
+    // bins by lane shuffles
+    constexpr bool SHUF = OP == OP_C2R_EVEN && !LF_IN && !FLAT && TPL >= 2 && TPL <= 64 &&
+                          StageShape<SC, N, TPL, 0>::NR == TPL;
+    fstage<T, N, TPL, OP, VIN, VOUT, DIRECT_OUT, SC, 0, FLAT, SHUF>(c);
   }
   if constexpr (FLAT) {
     // natural-order result of every line is in LDS (the last stage ended
@@ -481,7 +532,7 @@ __global__ void __launch_bounds__(TPL *LPB, SPLIT ? 4 : 1) k_fast(const LineArgs
       const int l2 = f / NOUT, k = f - l2 * NOUT;
       T *g = a.out + s_oo[l2];
       if constexpr (OP == OP_R2C_EVEN)
-        stc<true>(g + k * a.eso, r2c_bin(a, lds + l2 * P, k));
+        stc<true>(g + k * a.eso, r2c_bin(a, c.rtw, lds + l2 * P, k));
       else
         fstore<T, OP, true>(a, c.twl, g, s_aux[l2], k, lds[l2 * P + padi<T>(k)]);
     }
@@ -495,7 +546,7 @@ __global__ void __launch_bounds__(TPL *LPB, SPLIT ? 4 : 1) k_fast(const LineArgs
       const C2<T> *ln = lds + lo * P;
       T *g = a.out + s_oo[lo];
       if constexpr (OP == OP_R2C_EVEN) {
-        for (int k = io; k <= N; k += TPL) stc<VOUT>(g + k * a.eso, r2c_bin(a, ln, k));
+        for (int k = io; k <= N; k += TPL) stc<VOUT>(g + k * a.eso, r2c_bin(a, c.rtw, ln, k));
       } else {
         for (int k = io; k < N; k += TPL)
           fstore<T, OP, VOUT>(a, c.twl, g, s_aux[lo], k, ln[padi<T>(k)]);
@@ -515,6 +566,7 @@ void go(const LineArgs<T> &a0, hipStream_t s) {
   const void *fn = reinterpret_cast<const void *>(
       &k_fast<T, N, TPL, LPB, LF_IN, LF_OUT, OP, VIN, VOUT, SC, SPLIT>);
   size_t lds = lines_bytes<T, N, LPB, SPLIT>();
+  if constexpr (rtw_in_lds<T, N, OP>()) lds += size_t(N) * sizeof(C2<T>);
   LineArgs<T> a = a0;
   a.twlds_lo = a.twlds_hi = 0;
   if constexpr (OP == OP_PASSA) {
