@@ -168,25 +168,61 @@ __device__ __forceinline__ void fstore(const LineArgs<T> &a, const C2<T> *twl, T
   }
 }
 
+// r2c post-processing of bin k (1..h-1) from bins k and h - k of the
+// half-length spectrum z
+template <typename T>
+__device__ __forceinline__ C2<T> r2c_combine(const LineArgs<T> &a, const C2<T> *t0, C2<T> zk,
+                                             C2<T> zhk, int k) {
+  const C2<T> zm = conjc(zhk);
+  const C2<T> fe = T(0.5) * (zk + zm);
+  const C2<T> dd = zk - zm;
+  const C2<T> fo = C2<T>{T(0.5) * dd.y, T(-0.5) * dd.x};
+  C2<T> X = fe + cmul(t0[k], fo);
+  if (a.sub & SUB_CONJ) X.y = -X.y;
+  return a.fct * X;
+}
+
+// bins 0 and h, both from z_0
+template <typename T>
+__device__ __forceinline__ C2<T> r2c_edge(const LineArgs<T> &a, C2<T> z0, bool top) {
+  return a.fct * C2<T>{top ? z0.x - z0.y : z0.x + z0.y, T(0)};
+}
+
 // r2c post-processing of bin k (0..h) from the natural-order half-length
 // spectrum in LDS
 template <typename T>
 __device__ __forceinline__ C2<T> r2c_bin(const LineArgs<T> &a, const C2<T> *t0, const C2<T> *line,
                                          int k) {
   const int h = a.f.n;
-  C2<T> X;
-  if (k == 0 || k == h) {
-    const C2<T> z = line[0];
-    X = C2<T>{k == 0 ? z.x + z.y : z.x - z.y, T(0)};
-  } else {
-    const C2<T> zk = line[padi<T>(k)], zm = conjc(line[padi<T>(h - k)]);
-    const C2<T> fe = T(0.5) * (zk + zm);
-    const C2<T> dd = zk - zm;
-    const C2<T> fo = C2<T>{T(0.5) * dd.y, T(-0.5) * dd.x};
-    X = fe + cmul(t0[k], fo);
+  if (k == 0 || k == h) return r2c_edge(a, line[0], k == h);
+  return r2c_combine(a, t0, line[padi<T>(k)], line[padi<T>(h - k)], k);
+}
+
+// r2c from the registers of the last stage: thread i holds bins
+// k = (i + q TPL) + r NS (NS = Q TPL); bin N - k is slot (Q-1-q, R-1-r) of
+// lane TPL - i, or for i = 0 slot (Q-q, R-1-r) (q >= 1) / (0, R-r) (q = 0)
+// of the lane itself.  Every bin is stored straight from registers.
+template <typename T, int N, int TPL, int Q, int R, bool VOUT>
+__device__ __forceinline__ void r2c_store_shuffled(const FCtx<T> &c, const C2<T> (&v)[Q][R]) {
+  const LineArgs<T> &a = *c.a;
+  constexpr int NS = Q * TPL;
+  const int i = c.i;
+  const int lane = threadIdx.x & 63;
+  const int src = i == 0 ? lane : lane + TPL - 2 * i;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const C2<T> w = v[Q - 1 - q][R - 1 - r];
+      C2<T> m{__shfl(w.x, src), __shfl(w.y, src)};
+      if (i == 0) m = q == 0 ? v[0][(R - r) % R] : v[Q - q][R - 1 - r];
+      const int k = i + q * TPL + r * NS;
+      const C2<T> X = (i == 0 && q == 0 && r == 0) ? r2c_edge(a, v[0][0], false)
+                                                   : r2c_combine(a, c.rtw, v[q][r], m, k);
+      if (c.valid) stc<VOUT>(c.gout + k * a.eso, X);
+    }
   }
-  if (a.sub & SUB_CONJ) X.y = -X.y;
-  return a.fct * X;
+  if (c.valid && i == 0) stc<VOUT>(c.gout + N * a.eso, r2c_edge(a, v[0][0], true));
 }
 
 // c2r stage 0 with the bins h - p taken from the lanes that loaded them:
@@ -253,7 +289,11 @@ __device__ __forceinline__ void fstage(const FCtx<T> &c) {
       dft<R>(v[q], dir, a.gen);
     }
   }
-  if constexpr (LAST && DIRECT_OUT) {
+  if constexpr (LAST && DIRECT_OUT && OP == OP_R2C_EVEN) {
+    static_assert(Q * TPL == NR && NR == NS, "r2c from registers needs full butterflies");
+    r2c_store_shuffled<T, N, TPL, Q, R, VOUT>(c, v);
+    return;
+  } else if constexpr (LAST && DIRECT_OUT) {
     if (c.valid) {
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
@@ -437,7 +477,12 @@ __global__ void __launch_bounds__(TPL *LPB, SPLIT ? 4 : 1) k_fast(const LineArgs
   // more than they save.
   constexpr bool FLAT = !LF_IN && !LF_OUT && TPL == 1 && VIN && VOUT && !SPLIT &&
                         (OP == OP_C2C || OP == OP_R2C_EVEN || OP == OP_C2R_EVEN);
-  constexpr bool DIRECT_OUT = (LF_IN == LF_OUT) && OP != OP_R2C_EVEN && !FLAT;
+  // r2c stores from registers when a line's lanes are adjacent in the wave
+  // (mirrored bins by lane shuffles; the last stage covers every slot)
+  constexpr bool R2C_REG =
+      OP == OP_R2C_EVEN && !LF_IN && !LF_OUT && !SPLIT && TPL >= 2 && TPL <= 64 &&
+      StageShape<SC, N, TPL, SC::count - 1>::Q * TPL == StageShape<SC, N, TPL, SC::count - 1>::NR;
+  constexpr bool DIRECT_OUT = (LF_IN == LF_OUT) && (OP != OP_R2C_EVEN || R2C_REG) && !FLAT;
   C2<T> *lds = reinterpret_cast<C2<T> *>(smem_raw);
   T *slds = reinterpret_cast<T *>(smem_raw);
   __shared__ int64_t s_oo[LPB];
