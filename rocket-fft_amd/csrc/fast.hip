@@ -267,18 +267,19 @@ __device__ __forceinline__ void fstage(const FCtx<T> &c) {
   for (int q = 0; q < Q; ++q) {
     const int j = c.i + q * TPL;
     if (Q * TPL == NR || j < NR) {
-      if constexpr (SHUF) c2r_load_shuffled<T, N, TPL, R, VIN>(c, v[q]);
+      if constexpr (SHUF) {
+        c2r_load_shuffled<T, N, TPL, R, VIN>(c, v[q]);
+      } else {
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if constexpr (SHUF)
-          ;
-        else if constexpr (FIRST && !FLAT)
-          v[q][r] = c.valid ? fload<T, OP, VIN>(c, j + r * NR) : C2<T>{T(0), T(0)};
-        else if constexpr (FIRST && OP == OP_C2R_EVEN)
-          v[q][r] = c2r_pre(a, c.rtw, c.line[padi<T>(j + r * NR)], c.line[padi<T>(N - j - r * NR)],
-                            j + r * NR);
-        else
-          v[q][r] = c.line[padi<T>(j + r * NR)];
+        for (int r = 0; r < R; ++r) {
+          if constexpr (FIRST && !FLAT)
+            v[q][r] = c.valid ? fload<T, OP, VIN>(c, j + r * NR) : C2<T>{T(0), T(0)};
+          else if constexpr (FIRST && OP == OP_C2R_EVEN)
+            v[q][r] = c2r_pre(a, c.rtw, c.line[padi<T>(j + r * NR)],
+                              c.line[padi<T>(N - j - r * NR)], j + r * NR);
+          else
+            v[q][r] = c.line[padi<T>(j + r * NR)];
+        }
       }
       if constexpr (NS > 1) {
         const int k = j % NS;
