@@ -382,8 +382,8 @@ static uint64_t pick_split(uint64_t N, bool dbl, int lines, size_t budget) {
 
 static int two_pass_lines(bool dbl) { return dbl ? 4 : 8; }  // 64-byte row segments
 
-static void make_two_pass(C2CPlan &p, uint64_t N, size_t budget) {
-  const uint64_t n1 = pick_split(N, p.dbl, two_pass_lines(p.dbl), budget);
+static void make_two_pass(C2CPlan &p, uint64_t N, size_t budget, uint64_t n1_given = 0) {
+  const uint64_t n1 = n1_given ? n1_given : pick_split(N, p.dbl, two_pass_lines(p.dbl), budget);
   if (!n1) RFH_THROW(3, "no two-pass split for length " + std::to_string(N));
   p.n1 = n1;
   p.n2 = N / n1;
@@ -447,6 +447,32 @@ static void make_multi(C2CPlan &p, uint64_t N, uint64_t n1, size_t budget) {
   p.sub = get_c2c_plan_any(p.n2, p.dbl, budget, true);
 }
 
+// GPU Bluestein length: the reference pads to good_size(2n - 1)
+// (_pocketfft_hdronly.h:2765-2767), whose balanced two-pass split rarely has
+// a power-of-two side (131220 = 324 x 405 for n = 65537).  Any length
+// >= 2n - 1 gives the same transform, so when the padded length needs two
+// passes the GPU may instead use n1 * m with n1 a power of two (the column
+// passes on the specialised kernel) and m = good_size(ceil((2n - 1) / n1)),
+// if that is at most 6% longer.  Returns n1 (0: keep the reference length).
+static uint64_t pick_blue_pow2(uint64_t n, uint64_t ref_n2, bool dbl, size_t budget,
+                               uint64_t *gpu_n2) {
+  const uint64_t lim = direct_max(dbl, two_pass_lines(dbl), budget);
+  const uint64_t need = 2 * n - 1;
+  uint64_t best_n1 = 0, best = 0;
+  for (uint64_t n1 : {uint64_t(1024), uint64_t(512), uint64_t(256)}) {
+    if (n1 > lim || !fast_interleaved_ok(n1, dbl)) continue;
+    const uint64_t m = good_size_cmplx((need + n1 - 1) / n1);
+    if (m < 2 || m > lim) continue;
+    if (!best || n1 * m < best) {
+      best = n1 * m;
+      best_n1 = n1;
+    }
+  }
+  if (!best || double(best) > 1.06 * double(ref_n2)) return 0;
+  *gpu_n2 = best;
+  return best_n1;
+}
+
 static std::shared_ptr<C2CPlan> build_c2c_x(uint64_t n, bool dbl, size_t budget, bool inter) {
   auto p = std::make_shared<C2CPlan>();
   p->n = n;
@@ -483,9 +509,14 @@ static std::shared_ptr<C2CPlan> build_c2c_x(uint64_t n, bool dbl, size_t budget,
     blue = true;
     bn2 = good_size_cmplx(2 * n - 1);
   }
+  uint64_t gbn2 = 0, gn1 = 0;
   if (direct_ok(bn2)) {
     p->kind = kBlueDirect;
     make_direct(p->main, bn2, dbl);
+  } else if ((gn1 = pick_blue_pow2(n, bn2, dbl, budget, &gbn2)) != 0) {
+    p->kind = kBlueTwoPass;
+    bn2 = gbn2;
+    make_two_pass(*p, bn2, budget, gn1);
   } else if (pick_split(bn2, dbl, lines, budget)) {
     p->kind = kBlueTwoPass;
     make_two_pass(*p, bn2, budget);
