@@ -15,9 +15,9 @@
 // a power of two so its largest input is just below 2^10 (exact, undone
 // after the product), keeping FP16 in range.
 // BF16 form: bf16 has FP32's range but 8 mantissa bits, so operands are
-// split three ways (hi + mid + lo) and the six products down to 2^-24 are
-// accumulated; a two-way BF16 split (three products, ~16 bits) is kept as a
-// precision-study mode.
+// split three ways (hi + mid + lo), unscaled, and the six products down to
+// 2^-24 are accumulated; a two-way BF16 split (three products, ~16 bits) is
+// kept as a precision-study mode.
 //
 // Operand layout of v_mfma_f32_16x16x32_{f16,bf16} (64 lanes; lane l, g =
 // l / 16): A row l % 16, columns 8g..8g+7; B rows 8g..8g+7, column l % 16;
@@ -188,19 +188,26 @@ __global__ void __launch_bounds__(256, 4) k_fft4096_mfma(const LineArgs<float> a
     }
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-      float m = 0.f;
       C2<float> v[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         v[i] = in[g][i];
         if (s > 0) v[i] = cmul(v[i], dirw(w[g][i], dir));
-        m = fmaxf(m, fmaxf(fabsf(v[i].x), fabsf(v[i].y)));
       }
-      m = group_max(m);
-      int ex = 0;
-      (void)frexpf(m, &ex);
-      const float sc = m > 0.f ? ldexpf(1.f, 10 - ex) : 1.f;
-      inv[g] = m > 0.f ? ldexpf(1.f, ex - 10) : 1.f;  // exact: sc is a power of two
+      // FP16 needs each butterfly scaled into its range; BF16 has FP32's
+      // exponent range and is split unscaled
+      float sc = 1.f;
+      inv[g] = 1.f;
+      if constexpr (MODE == MODE_F16) {
+        float m = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) m = fmaxf(m, fmaxf(fabsf(v[i].x), fabsf(v[i].y)));
+        m = group_max(m);
+        int ex = 0;
+        (void)frexpf(m, &ex);
+        sc = m > 0.f ? ldexpf(1.f, 10 - ex) : 1.f;
+        inv[g] = m > 0.f ? ldexpf(1.f, ex - 10) : 1.f;  // exact: sc is a power of two
+      }
       // B rows 8 kg .. 8 kg + 7: the lower lane (l < 32) takes the real
       // parts of the eight shared inputs (its own four, then those of lane
       // l + 32), the upper lane the imaginary parts (those of lane l - 32,
