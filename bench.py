@@ -44,9 +44,9 @@ BYTES_C2C_PASS = 2 * N3 * N3 * (N3 // 2 + 1) * 8
 #   rocprofv3 --pmc FETCH_SIZE -- python3 bench.py --probe   (then WRITE_SIZE)
 # each counter scaled by what it reads for the device copy of known size
 # (67,633,152 B each way) in the same run: FETCH_SIZE x 1.999, WRITE_SIZE
-# x 1.000 on MI355X.  Mean of six launches: 67,788,775 B fetched +
-# 69,053,451 B written = 1.012 x the algorithmic bytes.
-TRAFFIC_C2C_PASS = 136842226
+# x 1.000 on MI355X.  Mean of six launches: 67,786,089 B fetched +
+# 68,978,544 B written = 1.011 x the algorithmic bytes.
+TRAFFIC_C2C_PASS = 136764633
 
 
 def rel_l2(a, b):
