@@ -11,9 +11,9 @@
 // lanes l and l + 32 then swap the halves the operand layout wants
 // (v_permlane32_swap), so no input is loaded or twiddled twice.
 // FP16 form: every operand is split x = hi + lo (two FP16 values, ~22 bits),
-// and D accumulates hi*hi + hi*lo + lo*hi; each butterfly is first scaled by
-// a power of two so its largest input is just below 2^10 (exact, undone
-// after the product), keeping FP16 in range.
+// and D accumulates hi*hi + hi*lo + lo*hi; the operands of stage s are first
+// scaled by 2^(10 - e - 4s), e the exponent of the line's largest input (a
+// power of two: exact, undone after the product), keeping FP16 in range.
 // BF16 form: bf16 has FP32's range but 8 mantissa bits, so operands are
 // split three ways (hi + mid + lo), unscaled, and the six products down to
 // 2^-24 are accumulated; a two-way BF16 split (three products, ~16 bits) is
@@ -116,20 +116,14 @@ struct Ops<MODE_BF16_2> : OpsBf<2> {};
 
 __device__ __forceinline__ int padf(int i) { return i + (i >> 4); }
 
-// Largest magnitude over the four lanes (l, l^16, l^32, l^48) that hold
-// one butterfly.
-__device__ __forceinline__ float group_max(float m) {
-  m = fmaxf(m, __shfl_xor(m, 16));
-  m = fmaxf(m, __shfl_xor(m, 32));
-  return m;
-}
-
 template <int MODE>
 __global__ void __launch_bounds__(256, 4) k_fft4096_mfma(const LineArgs<float> a) {
   using O = Ops<MODE>;
   typedef typename O::V V;
   constexpr int N = 4096, NR = 256;
   __shared__ C2<float> line[N + (N >> 4) + 1];
+  __shared__ float s_wmax[NR / 64];
+  int line_ex = 0;  // FP16: exponent of the line's largest input
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col = lane & 15, kg = lane >> 4;
   const bool upper = kg >= 2;  // lanes 32-63
@@ -186,6 +180,24 @@ __global__ void __launch_bounds__(256, 4) k_fft4096_mfma(const LineArgs<float> a
         w[g][i] = (s > 0 && r > 0) ? tw[(r - 1) * NS + k] : C2<float>{1.f, 0.f};
       }
     }
+    if constexpr (MODE == MODE_F16) {
+      if (s == 0) {  // one scale per line: the largest input, reduced over the block
+        float m = 0.f;
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) m = fmaxf(m, fmaxf(fabsf(in[g][i].x), fabsf(in[g][i].y)));
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if (lane == 0) s_wmax[wave] = m;
+        __syncthreads();
+        m = 0.f;
+#pragma unroll
+        for (int q = 0; q < NR / 64; ++q) m = fmaxf(m, s_wmax[q]);
+        (void)frexpf(m, &line_ex);
+        if (m == 0.f) line_ex = 10;
+      }
+    }
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
       C2<float> v[4];
@@ -194,19 +206,15 @@ __global__ void __launch_bounds__(256, 4) k_fft4096_mfma(const LineArgs<float> a
         v[i] = in[g][i];
         if (s > 0) v[i] = cmul(v[i], dirw(w[g][i], dir));
       }
-      // FP16 needs each butterfly scaled into its range; BF16 has FP32's
+      // FP16 needs the operands scaled into its range: stage s inputs are at
+      // most 16^s times the line's largest input, so 2^(10 - e - 4s) keeps
+      // them below 2^10 (exact, undone after the product); BF16 has FP32's
       // exponent range and is split unscaled
       float sc = 1.f;
       inv[g] = 1.f;
       if constexpr (MODE == MODE_F16) {
-        float m = 0.f;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) m = fmaxf(m, fmaxf(fabsf(v[i].x), fabsf(v[i].y)));
-        m = group_max(m);
-        int ex = 0;
-        (void)frexpf(m, &ex);
-        sc = m > 0.f ? ldexpf(1.f, 10 - ex) : 1.f;
-        inv[g] = m > 0.f ? ldexpf(1.f, ex - 10) : 1.f;  // exact: sc is a power of two
+        sc = ldexpf(1.f, 10 - line_ex - 4 * s);
+        inv[g] = ldexpf(1.f, line_ex - 10 + 4 * s);
       }
       // B rows 8 kg .. 8 kg + 7: the lower lane (l < 32) takes the real
       // parts of the eight shared inputs (its own four, then those of lane
