@@ -119,6 +119,18 @@ int rfh_set_fast_path(bool enabled);
  * hi/lo splits only (precision study; about 1e-5).  No reference
  * counterpart.  Returns the previous mode. */
 int rfh_set_precision(int mode);
+/* Number of matrix-core (split-precision) kernel launches since the library
+ * was loaded, in precision mode `mode` (1..3), or in all modes (0); lets a
+ * caller or test confirm that a transform actually ran on the matrix cores
+ * rather than on the FP32 butterflies.  No reference counterpart. */
+uint64_t rfh_mfma_launches(int mode);
+/* The DFT-16 operand table the matrix-core kernels of mode 1..3 load for
+ * direction -1 (forward) or +1: per split part t, per {A_re, A_im}, per lane
+ * l < 64, eight values ([(2 t + part) * 64 + l] * 8 + i), each exact in the
+ * operand format, generated from FP64 roots on the host.  Writes at most cap
+ * floats to out and returns the table's length (-1: bad mode or direction).
+ * Host-only (no GPU needed).  No reference counterpart. */
+int rfh_mfma_operands(int mode, int direction, float *out, int cap);
 /* One 16x16x32 matrix-core product on device buffers laid out as the
  * hardware operands (a, b: 64 lanes x 8 floats, d: 64 x 4; bf16 != 0 selects
  * BF16, else FP16), for tests of the operand layout. */

@@ -1028,5 +1028,9 @@ bool launch_mfma(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out,
                  bool vout, hipStream_t s);
 void mfma_probe(int bf, const float *a, const float *b, float *d, hipStream_t s);
 extern int g_precision_mode;
+// launches of the matrix-core kernels in precision mode `mode` (0: all modes)
+uint64_t mfma_launch_count(int mode);
+// the FP64-derived DFT-16 operand table of a mode and direction (host only)
+int mfma_operand_table(int mode, int dir, float *out, int cap);
 
 }  // namespace rfh

@@ -16,7 +16,10 @@
 //     H:4061-4091.
 // Each axis is one batched launch over all lines of that axis ("line set"),
 // never a loop over lines on the host.
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -24,6 +27,7 @@
 
 #include "../../include/rocket_fft_hip.h"
 #include "device.hpp"
+#include "workspace.hpp"
 
 namespace rfh {
 
@@ -41,6 +45,21 @@ void thread_stream(hipStream_t *s, bool *blocking) {
   *blocking = t_blocking;
 }
 
+namespace {
+std::atomic<pid_t> g_owner_pid{0};  // process that first used the library
+}  // namespace
+
+void fork_guard() {
+  const pid_t me = getpid();
+  pid_t owner = g_owner_pid.load();
+  if (owner == 0 && g_owner_pid.compare_exchange_strong(owner, me)) return;
+  if (owner != me)
+    RFH_THROW(4, "this process was forked from process " + std::to_string(owner) +
+                     " after that process had used the GPU through this library; HIP state "
+                     "does not survive fork(): use the library only in processes started "
+                     "fresh (spawn) or forked before the parent's first call");
+}
+
 void set_error(int code, const std::string &msg) {
   t_err = code;
   t_msg = msg;
@@ -51,25 +70,43 @@ void set_error(int code, const std::string &msg) {
 
 namespace {
 
+struct HipMem {
+  hipStream_t stream = nullptr;
+  void *alloc(size_t n) {
+    void *p = nullptr;
+    HIP_CHECK(hipMalloc(&p, n));
+    return p;
+  }
+  void free(void *p) { (void)hipFree(p); }
+  void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+};
+
+bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return cs != hipStreamCaptureStatusNone;
+}
+
 // Device scratch owned by one (device, stream): calls on the same stream are
-// serialised by `mu` so their temporaries cannot interleave.
+// serialised by `mu` so their temporaries cannot interleave.  Buffers follow
+// the graph-safe growth policy of WsSlot (workspace.hpp).
 struct Workspace {
   std::mutex mu;
-  DevBuf slot[6];
+  std::unique_ptr<WsSlot<HipMem>> slot[6];
   hipStream_t stream = nullptr;
+  bool capturing = false;  // set per call, with mu held
   void *get(int i, size_t bytes) {
-    if (bytes == 0) return nullptr;
-    DevBuf &b = slot[i];
-    if (b.bytes < bytes) {
-      const size_t want = std::max(bytes, b.bytes + b.bytes / 2);
-      HIP_CHECK(hipStreamSynchronize(stream));
-      if (b.p) HIP_CHECK(hipFree(b.p));
-      b.p = nullptr;
-      b.bytes = 0;
-      HIP_CHECK(hipMalloc(&b.p, want));
-      b.bytes = want;
-    }
-    return b.p;
+    if (!slot[i]) slot[i].reset(new WsSlot<HipMem>(HipMem{stream}));
+    WsSlot<HipMem>::Status st;
+    void *p = slot[i]->get(bytes, capturing, &st);
+    if (st == WsSlot<HipMem>::kGrowDuringCapture)
+      RFH_THROW(3, "the transform needs a larger device workspace (" + std::to_string(bytes) +
+                       " B) while the stream is capturing a graph; run the same call once "
+                       "before capture");
+    return p;
   }
 };
 
@@ -339,6 +376,7 @@ void with_device_arrays(Arr &in, size_t in_elem, Arr &out, size_t out_elem, size
   hipStream_t s = t_stream;
   Workspace &ws = workspace(s);
   std::lock_guard<std::mutex> lk(ws.mu);
+  ws.capturing = stream_capturing(s);
   const bool host = !din || !dout;
   char *in_host = in.data, *out_host = out.data;
   int64_t ilo = 0, ihi = 0, olo = 0, ohi = 0;
@@ -408,6 +446,7 @@ template <typename F>
 void guarded(F &&f) {
   try {
     t_err = 0;
+    fork_guard();
     f();
   } catch (const Error &e) {
     set_error(e.code, e.msg);
@@ -721,6 +760,12 @@ int rfh_mfma_probe(int bf16, const float *a, const float *b, float *d, void *str
   guarded([&] { mfma_probe(bf16, a, b, d, static_cast<hipStream_t>(stream)); });
   rc = t_err;
   return rc;
+}
+
+uint64_t rfh_mfma_launches(int mode) { return mfma_launch_count(mode); }
+
+int rfh_mfma_operands(int mode, int direction, float *out, int cap) {
+  return mfma_operand_table(mode, direction, out, cap);
 }
 
 int rfh_set_fast_path(bool enabled) {

@@ -470,10 +470,10 @@ namespace {
 // dimension to slice.
 constexpr size_t kChunkBytes = size_t(128) << 20;
 
-int64_t chunk_lines(const C2CPlan &p, const LineSet &g) {
+int64_t chunk_lines(const C2CPlan &p, const LineSet &g, size_t chunk_bytes) {
   if (p.kind != kTwoPass && p.kind != kBlueTwoPass) return g.nlines;
   const size_t per = size_t(p.n1) * p.n2 * (p.dbl ? 16 : 8);
-  const int64_t c = std::max<int64_t>(1, int64_t(kChunkBytes / per));
+  const int64_t c = std::max<int64_t>(1, int64_t((chunk_bytes ? chunk_bytes : kChunkBytes) / per));
   return (g.nb == 1 && g.nlines > c) ? c : g.nlines;
 }
 
@@ -503,16 +503,16 @@ int pass_lines(const C2CPlan &p, uint64_t n, int base) {
 
 }  // namespace
 
-size_t c2c_tmp_bytes(const C2CPlan &p, const LineSet &g) {
+size_t c2c_tmp_bytes(const C2CPlan &p, const LineSet &g, size_t chunk_bytes) {
   const size_t elem = p.dbl ? 16 : 8;
-  const size_t lines = size_t(chunk_lines(p, g));
+  const size_t lines = size_t(chunk_lines(p, g, chunk_bytes));
   switch (p.kind) {
     case kTwoPass: return lines * p.n1 * p.n2 * elem;
     case kBlueTwoPass: return lines * p.n1 * p.n2 * elem;
     case kMultiPass: {
       const size_t own = align256(size_t(g.nlines) * p.n1 * p.n2 * elem);
       const Tmp4 t = tmp4(g, p.n1, p.n2, nullptr, input_interleaved(g));
-      return own + c2c_tmp_bytes(*p.sub, multi_rows(g, p.n1, t));
+      return own + c2c_tmp_bytes(*p.sub, multi_rows(g, p.n1, t), chunk_bytes);
     }
     case kBlueMulti: {
       const size_t own = align256(size_t(g.nlines) * p.bn2 * elem);
@@ -523,21 +523,21 @@ size_t c2c_tmp_bytes(const C2CPlan &p, const LineSet &g) {
       const Work wl = work_layout(g, p.bn2, input_interleaved(g));
       for (int d = 0; d < g.nb; ++d) w.bsi[d] = w.bso[d] = wl.sb[d];
       w.esi = w.eso = wl.e;
-      return own + c2c_tmp_bytes(*p.sub, w);
+      return own + c2c_tmp_bytes(*p.sub, w, chunk_bytes);
     }
     default: return 0;
   }
 }
 
 std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct,
-                                     void *tmp) {
-  const int64_t cl = chunk_lines(p, g);
+                                     void *tmp, size_t chunk_bytes) {
+  const int64_t cl = chunk_lines(p, g, chunk_bytes);
   if (cl < g.nlines) {  // one two-pass sequence per chunk, sharing tmp
     std::vector<PassLaunch> out;
     const size_t scalar = p.dbl ? 8 : 4;
     for (int64_t c0 = 0; c0 < g.nlines; c0 += cl) {
       const LineSet s = slice_lines(g, c0, std::min(cl, g.nlines - c0), scalar);
-      for (const PassLaunch &pl : describe_c2c(p, s, dir, fct, tmp)) out.push_back(pl);
+      for (const PassLaunch &pl : describe_c2c(p, s, dir, fct, tmp, chunk_bytes)) out.push_back(pl);
     }
     return out;
   }
@@ -607,7 +607,8 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     a.lines = pass_lines(p, n1, lines);
     out.push_back(PassLaunch{a, pass_a(g, n1, M, g.in, g.esi, g.bsi, t)});
     void *rest = static_cast<char *>(tmp) + align256(size_t(g.nlines) * n1 * M * elem);
-    for (const PassLaunch &pl : describe_c2c(*p.sub, multi_rows(g, n1, t), dir, fct, rest))
+    for (const PassLaunch &pl :
+         describe_c2c(*p.sub, multi_rows(g, n1, t), dir, fct, rest, chunk_bytes))
       out.push_back(pl);
   } else if (p.kind == kBlueTwoPass) {
     // Bluestein in three passes over one intermediate t (n1 x n2 per line):
@@ -682,11 +683,13 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     e.op = OP_BLUE_IN;
     e.other = int64_t(N);
     out.push_back(PassLaunch{e, gin});
-    for (const PassLaunch &pl : describe_c2c(*p.sub, w, -1, 1.0, rest)) out.push_back(pl);
+    for (const PassLaunch &pl : describe_c2c(*p.sub, w, -1, 1.0, rest, chunk_bytes))
+      out.push_back(pl);
     e.op = OP_BLUE_MUL;
     e.other = int64_t(N);
     out.push_back(PassLaunch{e, w});
-    for (const PassLaunch &pl : describe_c2c(*p.sub, w, +1, 1.0, rest)) out.push_back(pl);
+    for (const PassLaunch &pl : describe_c2c(*p.sub, w, +1, 1.0, rest, chunk_bytes))
+      out.push_back(pl);
     e.op = OP_BLUE_OUT;
     e.fct = fct;
     e.other = int64_t(p.n);
@@ -717,9 +720,9 @@ void check_tmp_spans(const std::vector<PassLaunch> &passes, const void *tmp, siz
 }
 
 void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStream_t s,
-                void *tmp) {
-  const auto passes = describe_c2c(p, g, dir, fct, tmp);
-  check_tmp_spans(passes, tmp, c2c_tmp_bytes(p, g), p.dbl ? 8 : 4);
+                void *tmp, size_t chunk_bytes) {
+  const auto passes = describe_c2c(p, g, dir, fct, tmp, chunk_bytes);
+  check_tmp_spans(passes, tmp, c2c_tmp_bytes(p, g, chunk_bytes), p.dbl ? 8 : 4);
   for (const PassLaunch &pl : passes) run_pass_any(p.dbl, pl.ps, pl.g, s);
 }
 

@@ -121,6 +121,7 @@ extern "C" int rfh_roll(const void *in, void *out, int ndim, const int64_t *shap
                         void *stream) {
   using namespace rfh;
   try {
+    fork_guard();
     if (ndim < 0 || ndim > kRollDims) RFH_THROW(1, "rfh_roll: ndim out of range");
     RollGeom g{};
     g.ndim = ndim;
@@ -173,6 +174,7 @@ extern "C" int rfh_line_scale(void *data, int64_t nlines, int64_t len, int64_t s
                               const double *w, int mode, bool double_precision, void *stream) {
   using namespace rfh;
   try {
+    fork_guard();
     if (mode != 0 && mode != 1) RFH_THROW(1, "rfh_line_scale: mode must be 0 or 1");
     if (nlines < 0 || len < 0 || (nlines > 1 && stride < len))
       RFH_THROW(1, "rfh_line_scale: invalid geometry");

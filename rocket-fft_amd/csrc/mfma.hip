@@ -28,6 +28,14 @@
 // threads per line, one line per workgroup); the global loads of stage 0,
 // LDS exchanges with the padded layout, twiddle tables and stores of the
 // last stage follow the specialised kernel of fast.hip.
+#include <atomic>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
 #include "passes.hpp"
 
 namespace rfh {
@@ -85,6 +93,13 @@ struct Ops<MODE_F16> {
   __device__ static f4 mma(V a, V b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
   }
+  // eight values that are exact in FP16
+  __device__ static V load(const float *p) {
+    V v;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = (_Float16)p[i];
+    return v;
+  }
 };
 
 template <int TERMS>
@@ -108,6 +123,13 @@ struct OpsBf {
   __device__ static f4 mma(V a, V b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
+  // eight values that are exact in BF16
+  __device__ static V load(const float *p) {
+    V v;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = (__bf16)p[i];
+    return v;
+  }
 };
 template <>
 struct Ops<MODE_BF16_3> : OpsBf<3> {};
@@ -117,7 +139,8 @@ struct Ops<MODE_BF16_2> : OpsBf<2> {};
 __device__ __forceinline__ int padf(int i) { return i + (i >> 4); }
 
 template <int MODE>
-__global__ void __launch_bounds__(256, 4) k_fft4096_mfma(const LineArgs<float> a) {
+__global__ void __launch_bounds__(256, 4) k_fft4096_mfma(const LineArgs<float> a,
+                                                         const float *__restrict__ aop) {
   using O = Ops<MODE>;
   typedef typename O::V V;
   constexpr int N = 4096, NR = 256;
@@ -136,25 +159,14 @@ __global__ void __launch_bounds__(256, 4) k_fft4096_mfma(const LineArgs<float> a
   const float *gin = a.in + b * a.bsi[a.nb > 0 ? a.nb - 1 : 0];
   float *gout = a.out + b * a.bso[a.nb > 0 ? a.nb - 1 : 0];
 
-  // A operands: row = output k (lane % 16), columns 8 kg .. 8 kg + 7 of
-  // [Re | Im]: A_re = [Re F, -Im F], A_im = [Im F, Re F]
+  // A operands (row = output k = lane % 16, columns 8 kg .. 8 kg + 7 of
+  // [Re | Im]: A_re = [Re F, -Im F], A_im = [Im F, Re F]) from the FP64 host
+  // table of their split parts (each part exact in the operand format)
   V Are[O::terms], Aim[O::terms];
-  {
-    float re[8], im[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int kk = 8 * kg + i, r = kk & 15;
-      float sn, cs;
-      sincospif(float((col * r) & 15) / 8.0f, &sn, &cs);  // angle 2 pi k r / 16
-      const float fr = cs, fi = dir * sn;
-      re[i] = kk < 16 ? fr : -fi;
-      im[i] = kk < 16 ? fi : fr;
-    }
-#pragma unroll
-    for (int t = 0; t < O::terms; ++t) {
-      Are[t] = O::split(re, t);
-      Aim[t] = O::split(im, t);
-    }
+  for (int t = 0; t < O::terms; ++t) {
+    Are[t] = O::load(aop + ((2 * t) * 64 + lane) * 8);
+    Aim[t] = O::load(aop + ((2 * t + 1) * 64 + lane) * 8);
   }
 
 #pragma unroll
@@ -274,7 +286,86 @@ __global__ void __launch_bounds__(256, 4) k_fft4096_mfma(const LineArgs<float> a
   }
 }
 
+// ---------------------------------------------------------- host side
+
+// x rounded to `bits` significant bits (ties to even), with a quantum of at
+// least 2^qmin (FP16's subnormal step), in FP64.
+double round_sig(double x, int bits, int qmin) {
+  if (x == 0.0) return 0.0;
+  int e = 0;
+  (void)std::frexp(x, &e);  // |x| = m 2^e, 0.5 <= m < 1
+  const int q = std::max(e - bits, qmin);
+  return std::ldexp(std::nearbyint(std::ldexp(x, -q)), q);
+}
+
+// A operands of the radix-16 DFT product for one mode and direction: per
+// lane l (col = l % 16, kg = l / 16) and i < 8, column kk = 8 kg + i of
+// [Re F | -Im F] (A_re) and [Im F | Re F] (A_im), F[k][r] =
+// e^{dir 2 pi i k r / 16} in FP64, split into `terms` parts each exact in
+// the operand format (p_t = round(v - p_0 - ... - p_{t-1})), stored as
+// floats at [(2 t + {0: re, 1: im}) * 64 + l] * 8 + i.
+std::vector<float> dft16_operands(int mode, int dir) {
+  const int terms = mode == MODE_F16 ? 2 : (mode == MODE_BF16_3 ? 3 : 2);
+  const int bits = mode == MODE_F16 ? 11 : 8;
+  const int qmin = mode == MODE_F16 ? -24 : -133;
+  std::vector<float> t(size_t(terms) * 2 * 64 * 8);
+  for (int l = 0; l < 64; ++l) {
+    const int col = l & 15, kg = l >> 4;
+    for (int i = 0; i < 8; ++i) {
+      const int kk = 8 * kg + i, r = kk & 15, m = (col * r) & 15;
+      double cs = std::cos(2.0 * M_PI * m / 16.0), sn = std::sin(2.0 * M_PI * m / 16.0);
+      if (std::fabs(cs) < 1e-12) cs = 0.0;
+      if (std::fabs(sn) < 1e-12) sn = 0.0;
+      const double fr = cs, fi = dir * sn;
+      const double v[2] = {kk < 16 ? fr : -fi, kk < 16 ? fi : fr};
+      for (int c = 0; c < 2; ++c) {
+        double rest = v[c];
+        for (int k = 0; k < terms; ++k) {
+          const double p = round_sig(rest, bits, qmin);
+          t[(size_t(2 * k + c) * 64 + l) * 8 + i] = float(p);
+          rest -= p;
+        }
+      }
+    }
+  }
+  return t;
+}
+
+std::mutex g_aop_mu;
+std::map<std::tuple<int, int, int>, std::unique_ptr<DevBuf>> g_aop;  // (device, mode, dir)
+
+const float *operands(int mode, int dir) {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_aop_mu);
+  auto &b = g_aop[std::make_tuple(dev, mode, dir)];
+  if (!b) {
+
+    const std::vector<float> t = dft16_operands(mode, dir);
+    std::unique_ptr<DevBuf> nb(new DevBuf());
+    nb->upload(t.data(), t.size() * sizeof(float));
+    b = std::move(nb);
+  }
+  return static_cast<const float *>(b->p);
+}
+
+std::atomic<uint64_t> g_mfma_launches[4];
+
 }  // namespace
+
+int mfma_operand_table(int mode, int dir, float *out, int cap) {
+  if (mode < 1 || mode > 3 || (dir != 1 && dir != -1)) return -1;
+  const std::vector<float> t = dft16_operands(mode, dir);
+  for (int i = 0; i < int(t.size()) && i < cap; ++i) out[i] = t[i];
+  return int(t.size());
+}
+
+uint64_t mfma_launch_count(int mode) {
+  if (mode < 0 || mode > 3) return 0;
+  if (mode == 0)
+    return g_mfma_launches[1].load() + g_mfma_launches[2].load() + g_mfma_launches[3].load();
+  return g_mfma_launches[mode].load();
+}
 
 // Complex64 lines of 4096, contiguous, one line per group: the MFMA form.
 bool launch_mfma(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out, bool vin,
@@ -282,14 +373,17 @@ bool launch_mfma(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out,
   if (!g_precision_mode || op != OP_C2C || a.f.n != 4096 || lf_in || lf_out || !vin || !vout)
     return false;
   if (a.f.nst != 3 || a.esi != 2 || a.eso != 2 || a.nb > 1) return false;
+  const int mode = g_precision_mode;
+  const float *aop = operands(mode, a.dir < 0 ? -1 : 1);
   const unsigned grid = unsigned(a.nlines);
-  switch (g_precision_mode) {
-    case MODE_F16: hipLaunchKernelGGL(k_fft4096_mfma<MODE_F16>, dim3(grid), dim3(256), 0, s, a); break;
-    case MODE_BF16_3: hipLaunchKernelGGL(k_fft4096_mfma<MODE_BF16_3>, dim3(grid), dim3(256), 0, s, a); break;
-    default: hipLaunchKernelGGL(k_fft4096_mfma<MODE_BF16_2>, dim3(grid), dim3(256), 0, s, a); break;
+  switch (mode) {
+    case MODE_F16: hipLaunchKernelGGL(k_fft4096_mfma<MODE_F16>, dim3(grid), dim3(256), 0, s, a, aop); break;
+    case MODE_BF16_3: hipLaunchKernelGGL(k_fft4096_mfma<MODE_BF16_3>, dim3(grid), dim3(256), 0, s, a, aop); break;
+    default: hipLaunchKernelGGL(k_fft4096_mfma<MODE_BF16_2>, dim3(grid), dim3(256), 0, s, a, aop); break;
   }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) RFH_THROW(2, std::string("mfma launch: ") + hipGetErrorString(e));
+  g_mfma_launches[mode].fetch_add(1);
   return true;
 }
 

@@ -35,7 +35,7 @@ struct PassLaunch {
 // The passes launch_c2c would run for plan p over line set g (tmp = its
 // workspace), in order.
 std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct,
-                                     void *tmp);
+                                     void *tmp, size_t chunk_bytes = 0);
 // Throws if a pass that reads or writes the workspace [tmp, tmp + bytes)
 // would reach past its end.
 void check_tmp_spans(const std::vector<PassLaunch> &passes, const void *tmp, size_t bytes,

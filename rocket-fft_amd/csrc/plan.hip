@@ -592,6 +592,16 @@ std::shared_ptr<C2CPlan> get_c2c_plan(uint64_t n, bool dbl, size_t budget, bool 
   return get_c2c_plan_any(n, dbl, budget, interleaved);
 }
 
+std::shared_ptr<C2CPlan> find_c2c_plan(uint64_t n, bool dbl, size_t budget, bool interleaved) {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  const PlanKey key =
+      std::make_tuple(dev, n, dbl, budget ? budget : lds_dynamic_budget(), interleaved);
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  auto it = g_c2c.find(key);
+  return it == g_c2c.end() ? nullptr : it->second;
+}
+
 std::shared_ptr<RealPlan> get_real_plan(uint64_t n, bool dbl, size_t budget) {
   return cached(g_real, n, dbl, budget, false,
                 [](uint64_t n_, bool d_, size_t b_, bool) { return build_real(n_, d_, b_); });

@@ -477,31 +477,62 @@ void autotune(Batch &bt, hipStream_t s) {
       }
     }
   }
+  // The candidates run on scratch copies of the job buffers, never on the
+  // caller's: a job may be in place or read another job's output, and
+  // creating a batch must not transform anything.
+  Batch tb;
+  tb.dbl = bt.dbl;
+  tb.jobs = bt.jobs;
+  DevBuf scratch;
+  {
+    const size_t rs = bt.dbl ? 8 : 4, cs = 2 * rs;
+    size_t total = 0;
+    std::vector<std::pair<size_t, size_t>> off(tb.jobs.size());
+    for (size_t i = 0; i < tb.jobs.size(); ++i) {
+      const rfh_job_t &j = tb.jobs[i];
+      const size_t h = size_t(j.n / 2 + 1);
+      const size_t in_b = j.kind == RFH_JOB_C2C ? j.n * cs : (j.kind == RFH_JOB_R2C ? j.n * rs : h * cs);
+      const size_t out_b = j.kind == RFH_JOB_C2C ? j.n * cs : (j.kind == RFH_JOB_R2C ? h * cs : j.n * rs);
+      off[i].first = total;
+      total = align_up(total + in_b, 256);
+      off[i].second = total;
+      total = align_up(total + out_b, 256);
+    }
+    if (total) {
+      HIP_CHECK(hipMalloc(&scratch.p, total));
+      scratch.bytes = total;
+      HIP_CHECK(hipMemsetAsync(scratch.p, 0, total, s));
+    }
+    char *base = static_cast<char *>(scratch.p);
+    for (size_t i = 0; i < tb.jobs.size(); ++i) {
+      tb.jobs[i].in = base + off[i].first;
+      tb.jobs[i].out = base + off[i].second;
+    }
+  }
   const size_t cap = lds_limit() - 8192;
   double best = 1e300;
   size_t best_budget = 0;
   int best_grid = 0;
   for (size_t budget : {size_t(32768), size_t(65536), std::min<size_t>(98304, cap)}) {
-    std::unique_ptr<Built> cand;
     try {
-      cand = build_any(bt, budget);
+      tb.b = build_any(tb, budget);
     } catch (const Error &) {
       continue;
     }
-    std::swap(bt.b, cand);
-    const int per_cu = std::max(1, int((lds_limit() - 8192) / std::max<size_t>(bt.b->lds_need, 1)));
+    const int per_cu = std::max(1, int((lds_limit() - 8192) / std::max<size_t>(tb.b->lds_need, 1)));
     for (int wpc : {1, 2, 4}) {
       if (wpc > per_cu) continue;
       const int grid = cu_count() * wpc;
-      const double ms = time_ms(bt, grid, s, 3);
+      const double ms = time_ms(tb, grid, s, 3);
       if (ms < best) {
         best = ms;
         best_budget = budget;
         best_grid = grid;
       }
     }
-    std::swap(bt.b, cand);
   }
+  HIP_CHECK(hipStreamSynchronize(s));  // the timing runs are done with the scratch copies
+  tb.b.reset();
   if (!best_budget) RFH_THROW(3, "ragged batch: no LDS budget fits every job");
   bt.b = build_any(bt, best_budget);
   bt.grid = best_grid;
@@ -515,9 +546,23 @@ void autotune(Batch &bt, hipStream_t s) {
   }
 }
 
+// device or managed memory (the persistent kernel dereferences job pointers
+// directly; host memory would fault)
+bool device_memory(const void *p) {
+  if (!p) return false;
+  hipPointerAttribute_t at;
+  std::memset(&at, 0, sizeof(at));
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
 template <typename F>
 int guarded_int(F &&f) {
   try {
+    fork_guard();
     f();
     return 0;
   } catch (const Error &e) {
@@ -551,6 +596,10 @@ int rfh_batch_create(const rfh_job_t *jobs, int64_t njobs, bool double_precision
       if ((reinterpret_cast<uintptr_t>(j.in) | reinterpret_cast<uintptr_t>(j.out)) % al) {
         delete bt;
         RFH_THROW(3, "ragged batch: job buffers must be aligned to the complex element size");
+      }
+      if (j.n > 0 && (!device_memory(j.in) || !device_memory(j.out))) {
+        delete bt;
+        RFH_THROW(3, "ragged batch: job buffers must be device (or managed) memory");
       }
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -644,8 +693,7 @@ int rfh_batch_info(void *handle, int *grid, int *lds_bytes, int *autotuned, doub
 }
 
 int rfh_batch_destroy(void *handle) {
-  delete static_cast<Batch *>(handle);
-  return 0;
+  return guarded_int([&] { delete static_cast<Batch *>(handle); });
 }
 
 }  // extern "C"

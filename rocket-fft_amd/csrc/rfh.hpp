@@ -33,6 +33,11 @@ struct Error {
   std::string msg;
 };
 #define RFH_THROW(code, msg) throw ::rfh::Error{(code), (msg)}
+// Every entry point that may touch HIP calls this first.  The first call
+// records the process; in a child forked after that, HIP state inherited
+// from the parent is unusable, so the call fails with code 4 before any HIP
+// call is made (a child of a parent that never used the library works).
+void fork_guard();
 #define HIP_CHECK(expr)                                                              \
   do {                                                                               \
     hipError_t e_ = (expr);                                                          \
@@ -130,6 +135,10 @@ bool fast_interleaved_ok(uint64_t n, bool dbl);
 std::shared_ptr<C2CPlan> get_c2c_plan(uint64_t n, bool dbl, size_t lds_budget = 0,
                                       bool interleaved = false);
 std::shared_ptr<RealPlan> get_real_plan(uint64_t n, bool dbl, size_t lds_budget = 0);
+// The cached plan of get_c2c_plan's arguments on the current device, or
+// null when none was built yet (never allocates).
+std::shared_ptr<C2CPlan> find_c2c_plan(uint64_t n, bool dbl, size_t lds_budget = 0,
+                                       bool interleaved = false);
 
 // Radix schedule (execution order) for an LDS-resident FFT of length n.
 std::vector<int> gpu_radices(uint64_t n);
@@ -163,9 +172,12 @@ struct LineSet {
 
 // Kernel launchers (kernels.hip).  All enqueue on `s`.
 // dir: -1 forward, +1 backward.
+// chunk_bytes: largest four-step intermediate per chunk of lines (0: the
+// default, sized to stay in the Infinity Cache); tmp must hold
+// c2c_tmp_bytes(p, g, chunk_bytes).
 void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStream_t s,
-                void *tmp);
-size_t c2c_tmp_bytes(const C2CPlan &p, const LineSet &g);
+                void *tmp, size_t chunk_bytes = 0);
+size_t c2c_tmp_bytes(const C2CPlan &p, const LineSet &g, size_t chunk_bytes = 0);
 
 enum RealOp { kR2C = 0, kC2R = 1 };
 // Fused real transforms: r2c (real lines -> n/2+1 complex, conj when

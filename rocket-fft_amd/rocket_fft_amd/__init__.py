@@ -326,6 +326,15 @@ def set_precision(mode):
     return {v: k for k, v in PRECISION_MODES.items()}[prev]
 
 
+def mfma_launches(mode=None):
+    """Matrix-core kernel launches so far in `mode` ("mfma_fp16", "mfma_bf16",
+    "mfma_bf16_2split"), or in all modes (None) (rfh_mfma_launches)."""
+    L = _load()
+    L.rfh_mfma_launches.argtypes = [ctypes.c_int]
+    L.rfh_mfma_launches.restype = ctypes.c_uint64
+    return int(L.rfh_mfma_launches(0 if mode is None else PRECISION_MODES[mode]))
+
+
 def mfma_probe(a_lanes, b_lanes, bf16=False):
     """d (64 x 4) = one 16x16x32 matrix-core product of device operands laid
     out per lane (64 x 8 each), float32 torch CUDA tensors."""
@@ -435,8 +444,34 @@ class RaggedBatch:
         self._lib = L
         self._keep = []
         arr = (_Job * max(1, len(jobs)))()
+        device = None
         for i, (kind, forward, n, inp, out, fct) in enumerate(jobs):
             _check_bool("forward", forward)
+            if kind not in JOB_KINDS:
+                raise ValueError(f"job {i}: unknown kind {kind!r}")
+            _check_int("n", n)
+            if n < 0:
+                raise ValueError(f"job {i}: negative length")
+            rdt = torch.float64 if double else torch.float32
+            cdt = torch.complex128 if double else torch.complex64
+            h = n // 2 + 1
+            want_in = {"c2c": (cdt, n), "r2c": (rdt, n), "c2r": (cdt, h)}[kind]
+            want_out = {"c2c": (cdt, n), "r2c": (cdt, h), "c2r": (rdt, n)}[kind]
+            for name, t, (dt, count) in (("input", inp, want_in), ("output", out, want_out)):
+                if not (_is_torch(t) and t.is_cuda):
+                    raise TypeError(f"job {i}: {name} must be a CUDA tensor")
+                if device is None:
+                    device = t.device
+                elif t.device != device:
+                    raise ValueError(f"job {i}: {name} is on {t.device}, the batch on {device}")
+                if t.dtype != dt:
+                    raise TypeError(f"job {i} ({kind}, double={bool(double)}): {name} must be "
+                                    f"{dt}, got {t.dtype}")
+                if not t.is_contiguous():
+                    raise ValueError(f"job {i}: {name} must be contiguous")
+                if t.numel() < count:
+                    raise ValueError(f"job {i}: {name} holds {t.numel()} elements, "
+                                     f"the transform needs {count}")
             arr[i].kind = JOB_KINDS[kind]
             arr[i].forward = int(bool(forward))
             arr[i].n = int(n)

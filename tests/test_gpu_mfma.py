@@ -5,6 +5,9 @@
 * complex64 transforms of length 4096 in each precision mode against the
   long-double oracle on both input sets of the precision configuration:
   iid N(0,1), and magnitudes log-uniform over 1e-3..1e3 with uniform phase.
+  Every matrix-core case asserts through the launch counter that the
+  matrix-core kernel ran (no silent FP32 fallback), and the two-way BF16
+  study split must be measurably less accurate than FP32.
 """
 import numpy as np
 import pytest
@@ -52,20 +55,29 @@ def inputs(kind, rows, rs):
 def test_mfma_4096_vs_fp64(mode, tol, kind):
     rs = np.random.RandomState(4)
     x = inputs(kind, 32, rs)
-    prev = R.set_precision(mode)
-    try:
-        for fwd, fct in ((True, 1.0), (False, 1.0 / 4096)):
-            t = torch.from_numpy(x).cuda()
-            y = torch.empty_like(t)
-            R.c2c(t, y, np.array([1]), fwd, fct, 1)
-            torch.cuda.synchronize()
-            want = oracle_out("c2c", x.astype(np.complex128), x.astype(np.complex128),
-                              np.array([1]), fwd, fct)
-            e = rel_l2(y.cpu().numpy(), want)
-            print(f"mode={mode} input={kind} forward={fwd}: rel-L2 vs FP64 = {e:.3e}", flush=True)
-            assert e <= tol
-    finally:
-        R.set_precision(prev)
+    errs = {}
+    for m in ("fp32", mode):
+        prev = R.set_precision(m)
+        try:
+            for fwd, fct in ((True, 1.0), (False, 1.0 / 4096)):
+                t = torch.from_numpy(x).cuda()
+                y = torch.empty_like(t)
+                before = R.mfma_launches(None if m == "fp32" else m)
+                R.c2c(t, y, np.array([1]), fwd, fct, 1)
+                torch.cuda.synchronize()
+                ran = R.mfma_launches(None if m == "fp32" else m) - before
+                assert ran == (0 if m == "fp32" else 1), f"mode {m}: {ran} matrix-core launches"
+                want = oracle_out("c2c", x.astype(np.complex128), x.astype(np.complex128),
+                                  np.array([1]), fwd, fct)
+                e = rel_l2(y.cpu().numpy(), want)
+                errs[(m, fwd)] = e
+                print(f"mode={m} input={kind} forward={fwd}: rel-L2 vs FP64 = {e:.3e}", flush=True)
+                assert e <= (1e-6 if m == "fp32" else tol)
+        finally:
+            R.set_precision(prev)
+    if mode == "mfma_bf16_2split":  # ~16-bit operands: clearly worse than FP32
+        for fwd in (True, False):
+            assert errs[(mode, fwd)] > 5 * errs[("fp32", fwd)]
 
 
 def test_mfma_mode_only_touches_4096():
@@ -76,8 +88,10 @@ def test_mfma_mode_only_touches_4096():
     try:
         t = torch.from_numpy(x).cuda()
         y = torch.empty_like(t)
+        before = R.mfma_launches()
         R.c2c(t, y, np.array([1]), True, 1.0, 1)
         torch.cuda.synchronize()
-        assert rel_l2(y.cpu().numpy(), np.fft.fft(x.astype(np.complex128), axis=1)) < 1e-6
+        assert R.mfma_launches() == before
+        assert rel_l2(y.cpu().numpy(), np.fft.fft(x.astype(np.complex128), axis=1)) <= 1e-6
     finally:
         R.set_precision(prev)

@@ -49,8 +49,8 @@ def test_c2c_two_pass_long(dtype):
 def test_c2c_bluestein_two_pass(dtype):
     rs = np.random.RandomState(8)
     x = crandn(rs, (2, 65537), dtype)
-    check("c2c", x, x, np.array([1]), True, 1.0, tol=3 * tol_of(dtype))
-    check("c2c", x, x, np.array([1]), False, 1.0 / 65537, tol=3 * tol_of(dtype))
+    check("c2c", x, x, np.array([1]), True, 1.0)
+    check("c2c", x, x, np.array([1]), False, 1.0 / 65537)
 
 
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
@@ -67,13 +67,13 @@ def test_bluestein_two_pass_pow2_columns(dtype):
     assert kind == 3 and n1 * n2 == 100352 and n1 & (n1 - 1) == 0 and n1 <= 4096
     rs = np.random.RandomState(9)
     x = crandn(rs, (2, n), dtype)
-    check("c2c", x, x, np.array([1]), True, 1.0, tol=3 * tol_of(dtype))
-    check("c2c", x, x, np.array([1]), False, 1.0 / n, tol=3 * tol_of(dtype))
+    check("c2c", x, x, np.array([1]), True, 1.0)
+    check("c2c", x, x, np.array([1]), False, 1.0 / n)
     xr = rrandn(rs, (2, n), np.float64 if dbl else np.float32)
     yr = np.zeros((2, n // 2 + 1), dtype)
-    check("r2c", xr, yr, np.array([1]), True, 1.0, tol=3 * tol_of(dtype))
+    check("r2c", xr, yr, np.array([1]), True, 1.0)
     xc = crandn(rs, (2, n // 2 + 1), dtype)
-    check("c2r", xc, xr, np.array([1]), False, 1.0 / n, tol=3 * tol_of(dtype))
+    check("c2r", xc, xr, np.array([1]), False, 1.0 / n)
 
 
 @pytest.mark.parametrize("shape,axes", [((16, 24, 10), (0, 1, 2)), ((16, 24, 10), (2, 0)),
@@ -251,7 +251,7 @@ def test_dct_long_unfused(kind, type, n, dtype):
     """Work lengths that need Bluestein or two passes run pre -> c2c -> post."""
     rs = np.random.RandomState(n)
     x = rrandn(rs, (3, n), dtype)
-    check(kind, x, x, np.array([1]), type, 1.0, True, tol=2 * tol_of(dtype))
+    check(kind, x, x, np.array([1]), type, 1.0, True)
 
 
 def test_separable_hartley_strided_axis0():
@@ -277,7 +277,7 @@ def test_two_pass_chunked(rows, n, dtype):
     ref = torch.fft.fft(x.to(torch.complex128), dim=1)
     torch.cuda.synchronize()
     e = float(torch.linalg.norm(y.to(torch.complex128) - ref) / torch.linalg.norm(ref))
-    assert e <= 3 * tol_of(dtype), e
+    assert e <= tol_of(dtype), e
 
 
 @pytest.mark.parametrize("shape,dtype", [((8192, 40), np.complex64), ((5000, 33), np.complex64),
@@ -289,9 +289,8 @@ def test_strided_long_axis(shape, dtype):
     as four-step passes with a line-fastest intermediate (4099: Bluestein)."""
     rs = np.random.RandomState(shape[0] % 1000)
     x = crandn(rs, shape, dtype)
-    tol = 3 * tol_of(dtype)
-    check("c2c", x, x, np.array([0]), True, 1.0, tol=tol)
-    check("c2c", x, x, np.array([0]), False, 1.0 / shape[0], tol=tol)
+    check("c2c", x, x, np.array([0]), True, 1.0)
+    check("c2c", x, x, np.array([0]), False, 1.0 / shape[0])
 
 
 @pytest.mark.parametrize("n,rows,dtype,kind", [
@@ -307,7 +306,7 @@ def test_multi_pass_lengths(n, rows, dtype, kind):
     assert k == kind
     rs = np.random.RandomState(n % 997)
     x = crandn(rs, (rows, n), dtype)
-    tol = 5 * tol_of(dtype)
+    tol = tol_of(dtype)
     want = np.fft.fft(x.astype(np.complex128), axis=1)
     got = on_gpu(R, "c2c", x, x, np.array([1]), True, 1.0)
     assert rel_l2(got, want) <= tol
