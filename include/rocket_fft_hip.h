@@ -111,6 +111,21 @@ int rfh_set_stream(void *stream, bool blocking);
  * kernels.  Process-wide; meant for tests that compare the two paths. */
 int rfh_set_fast_path(bool enabled);
 
+/* ---- allocation-free variant --------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): numba_c2c's transform with the same arguments (nthreads
+ * dropped), on device (or managed) memory of one GPU only, enqueued on
+ * `stream` without synchronising.  Its four-step intermediates live in a
+ * 64 MiB static device array of the library (one per GPU, held until the
+ * process ends) instead of an allocated workspace; a transform needing more
+ * than that is rejected with an error.  Plans and matrix-core tables are
+ * taken from the caches only: run numba_c2c once with the same shape (and
+ * precision mode) first, else the call fails.  Calls that overlap on one GPU
+ * share the array and race.  Returns 0, or the error code of rfh_last_error. */
+int rfh_c2c_noalloc(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout, arystruct_t *axes,
+                    bool forward, double fct, void *stream);
+
 /* ---- split-precision form ---------------------------------------------------
  * Process-wide arithmetic of complex64 transforms of length 4096 on
  * contiguous lines: 0 = plain FP32 butterflies (default), 1 = radix-16 stages

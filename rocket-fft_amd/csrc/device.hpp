@@ -1030,6 +1030,9 @@ void mfma_probe(int bf, const float *a, const float *b, float *d, hipStream_t s)
 extern int g_precision_mode;
 // launches of the matrix-core kernels in precision mode `mode` (0: all modes)
 uint64_t mfma_launch_count(int mode);
+// while on (per thread), a missing operand table is an error instead of an
+// allocation
+void mfma_no_alloc(bool on);
 // the FP64-derived DFT-16 operand table of a mode and direction (host only)
 int mfma_operand_table(int mode, int dir, float *out, int cap);
 

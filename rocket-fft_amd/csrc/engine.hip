@@ -672,6 +672,97 @@ void hartley(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout, arystruct
 }
 
 }  // namespace ops
+
+// ------------------------------------------------- allocation-free variant
+
+// Scratch of the allocation-free c2c entry point: the four-step
+// intermediates live in this file-scope device array instead of a
+// workspace allocated by the library.  Each GPU holds its own copy for the
+// life of the process; its device address is looked up once per device.
+constexpr size_t kStaticScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on device `dev` (current).
+void *scratch_on(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) RFH_THROW(3, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      RFH_THROW(2, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+
+struct NoAllocScope {
+  NoAllocScope() { mfma_no_alloc(true); }
+  ~NoAllocScope() { mfma_no_alloc(false); }
+};
+}  // namespace
+
+namespace ops {
+
+void c2c_noalloc(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout, arystruct_t *axes,
+                 bool forward, double fct, hipStream_t s) {
+  const bool dbl = ain->itemsize == 16;
+  const size_t sc = dbl ? 8 : 4;
+  const int64_t *shape = ain->shape_and_strides;
+  if (any_zero(shape, int(ndim))) return;
+  std::vector<uint64_t> ax;
+  check_axes(axes, ndim, ax);
+  if (ax.empty()) return;
+  Arr in = view(ndim, ain, shape, sc), out = view(ndim, aout, aout->shape_and_strides, sc);
+  check_fits(out, shape, "output");
+  for (int d = 0; d < out.ndim; ++d) out.shape[d] = shape[d];
+  int dev_in = -1, dev_out = -1;
+  if (!is_device_ptr(in.data, &dev_in) || !is_device_ptr(out.data, &dev_out) ||
+      dev_in != dev_out)
+    RFH_THROW(3, "the allocation-free c2c takes device (or managed) memory of one GPU only");
+  int prev = 0;
+  HIP_CHECK(hipGetDevice(&prev));
+  if (prev != dev_out) HIP_CHECK(hipSetDevice(dev_out));
+  struct Restore {
+    int d, p;
+    ~Restore() {
+      if (d != p) (void)hipSetDevice(p);
+    }
+  } restore{dev_out, prev};
+  void *tmp = scratch_on(dev_out);
+  NoAllocScope no_alloc;
+  for (size_t i = 0; i < ax.size(); ++i) {
+    const int a = int(ax[i]);
+    const int64_t *si = i == 0 ? in.str : out.str;
+    LineSet g = lines_for_axis(in.shape, in.ndim, a, si, out.str, i == 0 ? in.data : out.data,
+                               out.data);
+    bool inter = false;
+    for (int d = 0; d < g.nb; ++d)
+      inter = inter || std::llabs(g.bsi[d]) < std::llabs(g.esi) ||
+              std::llabs(g.bso[d]) < std::llabs(g.eso);
+    const uint64_t n = uint64_t(in.shape[a]);
+    auto plan = find_c2c_plan(n, dbl, 0, inter);
+    if (!plan)
+      RFH_THROW(3, "no plan on this GPU yet for length " + std::to_string(n) +
+                       (dbl ? " complex128" : " complex64") +
+                       (inter ? " (strided axis)" : "") +
+                       ": run numba_c2c once with this shape first (plans are built, and "
+                       "allocated, by the ordinary entry point)");
+    const size_t need = c2c_tmp_bytes(*plan, g, kStaticScratchBytes);
+    if (need > kStaticScratchBytes)
+      RFH_THROW(3, "the transform needs " + std::to_string(need) +
+                       " B of scratch; the allocation-free entry point has " +
+                       std::to_string(kStaticScratchBytes) + " B");
+    launch_c2c(*plan, g, forward ? -1 : +1, i == 0 ? fct : 1.0, s, tmp, kStaticScratchBytes);
+  }
+}
+
+}  // namespace ops
 }  // namespace rfh
 
 // ================================================================== C ABI
@@ -728,6 +819,14 @@ void numba_r2r_separable_hartley(uint64_t ndim, const arystruct_t *ain, arystruc
 void numba_r2r_genuine_hartley(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout,
                                arystruct_t *axes, double fct, uint64_t /*nthreads*/) {
   guarded([&] { ops::hartley(ndim, ain, aout, axes, fct, true); });
+}
+
+int rfh_c2c_noalloc(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout, arystruct_t *axes,
+                    bool forward, double fct, void *stream) {
+  guarded([&] {
+    ops::c2c_noalloc(ndim, ain, aout, axes, forward, fct, static_cast<hipStream_t>(stream));
+  });
+  return t_err;
 }
 
 int rfh_last_error(char *buf, size_t len) {

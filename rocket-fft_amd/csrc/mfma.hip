@@ -333,6 +333,7 @@ std::vector<float> dft16_operands(int mode, int dir) {
 
 std::mutex g_aop_mu;
 std::map<std::tuple<int, int, int>, std::unique_ptr<DevBuf>> g_aop;  // (device, mode, dir)
+thread_local bool t_no_alloc = false;  // the calling entry point may not allocate
 
 const float *operands(int mode, int dir) {
   int dev = 0;
@@ -340,6 +341,9 @@ const float *operands(int mode, int dir) {
   std::lock_guard<std::mutex> lk(g_aop_mu);
   auto &b = g_aop[std::make_tuple(dev, mode, dir)];
   if (!b) {
+    if (t_no_alloc)
+      RFH_THROW(3, "the matrix-core tables of this precision mode are not on the device yet; "
+                   "run the ordinary entry point once in this mode first");
 
     const std::vector<float> t = dft16_operands(mode, dir);
     std::unique_ptr<DevBuf> nb(new DevBuf());
@@ -352,6 +356,8 @@ const float *operands(int mode, int dir) {
 std::atomic<uint64_t> g_mfma_launches[4];
 
 }  // namespace
+
+void mfma_no_alloc(bool on) { t_no_alloc = on; }
 
 int mfma_operand_table(int mode, int dir, float *out, int cap) {
   if (mode < 1 || mode > 3 || (dir != 1 && dir != -1)) return -1;

@@ -222,6 +222,37 @@ def c2c(ain, aout, axes, forward, fct, nthreads=1):
     return _call("numba_c2c", ain, aout, axes, f, s, t)
 
 
+def c2c_noalloc(ain, aout, axes, forward, fct):
+    """c2c that allocates no device memory (rfh_c2c_noalloc): device tensors
+    only, enqueued on torch's current stream without synchronising; its
+    four-step intermediates live in the library's static 64 MiB scratch array
+    on the GPU.  Plans come from the plan cache: run c2c once with the same
+    shape first.  Calls that overlap on one GPU share the scratch and race."""
+    S_in, nd, kin, cin = _struct(ain)
+    S_out, nd2, kout, cout = _struct(aout)
+    if nd != nd2:
+        raise TypeError("Input and output array must have the same number of dimensions")
+    if not (cin and cout):
+        raise TypeError("c2c_noalloc takes CUDA tensors only")
+    _check_bool("forward", forward)
+    _check_float("fct", fct)
+    S_ax, keep = _axes_struct(axes, nd)
+    L = _load()
+    import torch
+    L.rfh_c2c_noalloc.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_bool, ctypes.c_double,
+                                  ctypes.c_void_p]
+    L.rfh_c2c_noalloc.restype = ctypes.c_int
+    stream = torch.cuda.current_stream(aout.device).cuda_stream
+    L.rfh_clear_error()
+    rc = L.rfh_c2c_noalloc(nd, ctypes.byref(S_in), ctypes.byref(S_out), ctypes.byref(S_ax),
+                           bool(forward), float(fct), ctypes.c_void_p(stream))
+    del keep
+    if rc:
+        raise RuntimeError(f"rfh_c2c_noalloc: {last_error()[1]}")
+    return aout
+
+
 def r2c(ain, aout, axes, forward, fct, nthreads=1):
     """Real-to-half-complex DFT (_pocketfft_numba.cpp:91-109)."""
     f, s, t = _cmplx_args(forward, fct, nthreads)
