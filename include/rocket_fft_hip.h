@@ -111,6 +111,17 @@ int rfh_set_stream(void *stream, bool blocking);
  * kernels.  Process-wide; meant for tests that compare the two paths. */
 int rfh_set_fast_path(bool enabled);
 
+/* ---- measurement --------------------------------------------------------------
+ * No reference counterpart.  rfh_kernel_timing(true) brackets every line-kernel
+ * launch with HIP events on the stream it is launched on (not while a stream is
+ * being captured into a graph: leave it off then).  rfh_kernel_timing_report
+ * waits for the recorded launches, forgets them, and writes one line per kernel
+ * label: "family op n=<length> c64|c128 [strided]\t<launches>\t<total ms>\t
+ * <algorithmic bytes: each launch's lines read once and written once>".
+ * Returns the report's length (the text is truncated to len - 1). */
+int rfh_kernel_timing(bool enabled);
+int rfh_kernel_timing_report(char *buf, size_t len);
+
 /* ---- allocation-free variant --------------------------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, serving loops, harnesses that forbid

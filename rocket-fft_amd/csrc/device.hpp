@@ -3,6 +3,7 @@
 // argument block and the load/store helpers.
 #pragma once
 
+#include "lines.hpp"
 #include "rfh.hpp"
 
 namespace rfh {
@@ -809,19 +810,10 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
   // offsets of the group's lines; the slots of missing lines (the last
   // group of a line set) hold zeros and are never used to address memory
   for (int l = threadIdx.x; l < a.L; l += blockDim.x) {
-    int64_t rem = b0 + l, oi = 0, oo = 0;
-    int aux = 0;
-    for (int d = l < nl ? a.nb - 1 : -1; d >= 0; --d) {
-      const int64_t q = rem / a.bdim[d];
-      const int64_t idx = rem - q * a.bdim[d];
-      rem = q;
-      oi += idx * a.bsi[d];
-      oo += idx * a.bso[d];
-      if (d == a.auxdim) aux = int(idx);
-    }
-    s_oi[l] = oi;
-    s_oo[l] = oo;
-    s_aux[l] = aux;
+    const LineOffsets o = group_slot(b0, l, a.L, a.nlines, a.nb, a.bdim, a.bsi, a.bso, a.auxdim);
+    s_oi[l] = o.oi;
+    s_oo[l] = o.oo;
+    s_aux[l] = o.aux;
   }
   __syncthreads();
 

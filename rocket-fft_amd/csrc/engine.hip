@@ -863,6 +863,21 @@ int rfh_mfma_probe(int bf16, const float *a, const float *b, float *d, void *str
 
 uint64_t rfh_mfma_launches(int mode) { return mfma_launch_count(mode); }
 
+int rfh_kernel_timing(bool enabled) {
+  kernel_timing(enabled);
+  return 0;
+}
+
+int rfh_kernel_timing_report(char *buf, size_t len) {
+  std::string r;
+  guarded([&] { r = kernel_timing_report(); });
+  if (buf && len) {
+    std::strncpy(buf, r.c_str(), len - 1);
+    buf[len - 1] = 0;
+  }
+  return int(r.size());
+}
+
 int rfh_mfma_operands(int mode, int direction, float *out, int cap) {
   return mfma_operand_table(mode, direction, out, cap);
 }

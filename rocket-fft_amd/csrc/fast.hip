@@ -518,19 +518,10 @@ __global__ void __launch_bounds__(TPL *LPB, SPLIT ? 4 : 1) k_fast(const LineArgs
   c.sline = slds + l * P;
   c.valid = b < a.nlines;
   c.i = i;
-  int64_t oi = 0, oo = 0;
-  int aux = 0;
-  if (c.valid) {
-    int64_t rem = b;
-    for (int d = a.nb - 1; d >= 0; --d) {
-      const int64_t q = rem / a.bdim[d];
-      const int64_t idx = rem - q * a.bdim[d];
-      rem = q;
-      oi += idx * a.bsi[d];
-      oo += idx * a.bso[d];
-      if (d == a.auxdim) aux = int(idx);
-    }
-  }
+  const LineOffsets lo_ = group_slot(tile * LPB, l, LPB, a.nlines, a.nb, a.bdim, a.bsi, a.bso,
+                                     a.auxdim);
+  const int64_t oi = lo_.oi, oo = lo_.oo;
+  const int aux = lo_.aux;
   c.gin = a.in + oi;
   c.gout = a.out + oo;
   c.aux = aux;

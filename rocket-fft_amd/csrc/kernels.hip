@@ -15,15 +15,22 @@
 // (H = rocket_fft/_pocketfft_hdronly.h.)  Remaining real transforms (DCT/DST,
 // Hartley, FFTPACK halfcomplex and real lengths without a fused plan) use
 // k_pre -> complex lines -> k_post around the same c2c machinery.
+#include <atomic>
 #include <cmath>
+#include <cstdio>
+#include <map>
 #include <mutex>
 #include <set>
+#include <string>
+#include <vector>
 
 #include "passes.hpp"
 
 namespace rfh {
 namespace {
 using namespace dev;
+
+thread_local const char *t_last_family = "k_lines";  // kernel family of the last launch
 
 template <typename T, int OP, bool VIN, bool VOUT>
 __global__ void __launch_bounds__(kThreads) k_lines(const LineArgs<T> a) {
@@ -210,9 +217,85 @@ __global__ void __launch_bounds__(kThreads) k_blue_elem(const LineArgs<T> a, int
   }
 }
 
+// ---------------------------------------------------- kernel timing (opt-in)
+// With timing on (rfh_kernel_timing), every line-kernel launch is bracketed by
+// HIP events on the stream it runs on and filed under a label naming the
+// kernel family, operation, length, precision, layout and algorithmic bytes;
+// rfh_kernel_timing_report waits for the events and sums them per label.
+std::atomic<bool> g_timing{false};
+struct Timed {
+  std::string label;
+  hipEvent_t e0, e1;
+  double bytes;
+};
+std::mutex g_timed_mu;
+std::vector<Timed> g_timed;
+
+const char *op_name(int op) {
+  switch (op) {
+    case OP_C2C: return "c2c";
+    case OP_PASSA: return "passA";
+    case OP_PASSB: return "passB";
+    case OP_BLUE: return "blue";
+    case OP_BLUEROW: return "bluerow";
+    case OP_R2C_EVEN: return "r2c_even";
+    case OP_R2C_ODD: return "r2c_odd";
+    case OP_C2R_EVEN: return "c2r_even";
+    case OP_C2R_ODD: return "c2r_odd";
+    case OP_R2R: return "r2r";
+    default: return "elem";
+  }
+}
+
+// runs `launch` (returns the kernel family's name) between two events
+template <typename F>
+void timed_launch(hipStream_t s, const std::string &what, double bytes, F &&launch) {
+  if (!g_timing.load()) {
+    launch();
+    return;
+  }
+  Timed t;
+  t.bytes = bytes;
+  HIP_CHECK(hipEventCreate(&t.e0));
+  HIP_CHECK(hipEventCreate(&t.e1));
+  HIP_CHECK(hipEventRecord(t.e0, s));
+  const char *fam = launch();
+  HIP_CHECK(hipEventRecord(t.e1, s));
+  t.label = std::string(fam) + " " + what;
+  std::lock_guard<std::mutex> lk(g_timed_mu);
+  g_timed.push_back(t);
+}
+
+template <typename T>
+void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s);
+
+template <typename T>
+void run_pass_timed(const PassSpec &ps, const LineSet &g, hipStream_t s) {
+  if (!g_timing.load() || g.nlines <= 0) {
+    run_pass<T>(ps, g, s);
+    return;
+  }
+  const bool elem = ps.op == OP_BLUE_IN || ps.op == OP_BLUE_MUL || ps.op == OP_BLUE_OUT;
+  const int64_t cin = elem ? ps.other : ps.nin, cout = elem ? ps.other : ps.nout;
+  const double sc = sizeof(T);
+  const double bytes = double(g.nlines) * ((ps.real_in ? 1 : 2) * sc * double(cin) +
+                                           (ps.real_out ? 1 : 2) * sc * double(cout));
+  int64_t minstride = 0;
+  for (int d = 0; d < g.nb; ++d)
+    minstride = d == 0 ? int64_t(std::llabs(g.bsi[d])) : std::min<int64_t>(minstride, std::llabs(g.bsi[d]));
+  const bool inter = g.nb > 0 && (minstride < std::llabs(g.esi));
+  const std::string what = std::string(op_name(ps.op)) + " n=" + std::to_string(ps.d->desc.n) +
+                           (sizeof(T) == 8 ? " c128" : " c64") + (inter ? " strided" : "");
+  timed_launch(s, what, bytes, [&]() -> const char * {
+    run_pass<T>(ps, g, s);
+    return t_last_family;
+  });
+}
+
 template <typename T>
 void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
   if (g.nlines <= 0) return;
+  t_last_family = "k_lines";
   if (ps.op == OP_BLUE_IN || ps.op == OP_BLUE_MUL || ps.op == OP_BLUE_OUT) {
     LineArgs<T> a = make_line_args<T>(ps, g);
     dim3 grid;
@@ -221,6 +304,7 @@ void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
     grid.y = unsigned(std::max<int64_t>(1, std::min<int64_t>(g.nlines, 65535)));
     hipLaunchKernelGGL(k_blue_elem<T>, grid, dim3(kThreads), 0, s, a, ps.op);
     HIP_CHECK(hipGetLastError());
+    t_last_family = "k_blue_elem";
     return;
   }
   LineArgs<T> a = make_line_args<T>(ps, g);
@@ -237,8 +321,14 @@ void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
   else
     vout = pairs_aligned(g.out, sizeof(T), g.eso, g.bso, g.nb);
   if constexpr (sizeof(T) == 4)
-    if (launch_mfma(ps.op, a, a.lfast_in, a.lfast_out, vin, vout, s)) return;
-  if (launch_fast<T>(ps.op, a, a.lfast_in, a.lfast_out, vin, vout, s)) return;
+    if (launch_mfma(ps.op, a, a.lfast_in, a.lfast_out, vin, vout, s)) {
+      t_last_family = "mfma";
+      return;
+    }
+  if (launch_fast<T>(ps.op, a, a.lfast_in, a.lfast_out, vin, vout, s)) {
+    t_last_family = "k_fast";
+    return;
+  }
   switch (ps.op) {
     case OP_C2C: launch_var<T, OP_C2C>(a, lds, vin, vout, s); break;
     case OP_PASSA: launch_var<T, OP_PASSA>(a, lds, vin, vout, s); break;
@@ -255,9 +345,9 @@ void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
 
 void run_pass_any(bool dbl, const PassSpec &ps, const LineSet &g, hipStream_t s) {
   if (dbl)
-    run_pass<double>(ps, g, s);
+    run_pass_timed<double>(ps, g, s);
   else
-    run_pass<float>(ps, g, s);
+    run_pass_timed<float>(ps, g, s);
 }
 
 // contiguous complex line set with g's batch shape, lines of length len
@@ -980,6 +1070,40 @@ __global__ void __launch_bounds__(kThreads) k_hartley(const MirrorGeom g, int64_
 }
 
 }  // namespace
+
+void kernel_timing(bool on) { g_timing.store(on); }
+
+std::string kernel_timing_report() {
+  std::vector<Timed> done;
+  {
+    std::lock_guard<std::mutex> lk(g_timed_mu);
+    done.swap(g_timed);
+  }
+  struct Acc {
+    int n = 0;
+    double ms = 0, bytes = 0;
+  };
+  std::map<std::string, Acc> acc;
+  for (Timed &t : done) {
+    float ms = 0;
+    if (hipEventSynchronize(t.e1) == hipSuccess && hipEventElapsedTime(&ms, t.e0, t.e1) == hipSuccess) {
+      Acc &a = acc[t.label];
+      ++a.n;
+      a.ms += ms;
+      a.bytes += t.bytes;
+    }
+    (void)hipEventDestroy(t.e0);
+    (void)hipEventDestroy(t.e1);
+  }
+  std::string out;
+  char line[512];
+  for (auto &kv : acc) {
+    std::snprintf(line, sizeof line, "%s\t%d\t%.6f\t%.0f\n", kv.first.c_str(), kv.second.n,
+                  kv.second.ms, kv.second.bytes);
+    out += line;
+  }
+  return out;
+}
 
 void launch_sym_fill(const MirrorGeom &mg, bool dbl, void *out, hipStream_t s) {
   int64_t total = 1;

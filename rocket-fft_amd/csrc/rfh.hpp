@@ -179,6 +179,12 @@ void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStre
                 void *tmp, size_t chunk_bytes = 0);
 size_t c2c_tmp_bytes(const C2CPlan &p, const LineSet &g, size_t chunk_bytes = 0);
 
+// Opt-in per-launch timing of the line kernels (HIP events on the launching
+// stream); the report waits for the recorded launches and lists, one line per
+// kernel label, "label\tlaunches\ttotal ms\talgorithmic bytes".
+void kernel_timing(bool on);
+std::string kernel_timing_report();
+
 enum RealOp { kR2C = 0, kC2R = 1 };
 // Fused real transforms: r2c (real lines -> n/2+1 complex, conj when
 // conj_flag) or c2r (n/2+1 complex -> real lines, input conjugated when

@@ -357,6 +357,31 @@ def set_precision(mode):
     return {v: k for k, v in PRECISION_MODES.items()}[prev]
 
 
+def kernel_timing(enabled):
+    """Per-launch HIP-event timing of the line kernels on their own streams
+    (rfh_kernel_timing); off by default, must be off while capturing graphs."""
+    L = _load()
+    L.rfh_kernel_timing.argtypes = [ctypes.c_bool]
+    L.rfh_kernel_timing.restype = ctypes.c_int
+    L.rfh_kernel_timing(bool(enabled))
+
+
+def kernel_timing_report():
+    """{label: (launches, total_ms, algorithmic_bytes)} of the launches timed
+    since the last report (waits for them)."""
+    L = _load()
+    L.rfh_kernel_timing_report.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    L.rfh_kernel_timing_report.restype = ctypes.c_int
+    buf = ctypes.create_string_buffer(1 << 20)
+    if L.rfh_kernel_timing_report(buf, len(buf)) >= len(buf):
+        raise RuntimeError("kernel timing report truncated")
+    out = {}
+    for line in buf.value.decode().splitlines():
+        label, cnt, ms, nbytes = line.split("\t")
+        out[label] = (int(cnt), float(ms), float(nbytes))
+    return out
+
+
 def mfma_launches(mode=None):
     """Matrix-core kernel launches so far in `mode` ("mfma_fp16", "mfma_bf16",
     "mfma_bf16_2split"), or in all modes (None) (rfh_mfma_launches)."""

@@ -148,5 +148,5 @@ def test_config4_precision_forms(shape, kind):
             e = rel_l2(got, ref("c2c", x, x, ax, True, 1.0))
             print(f"  fp32 vs reference complex64 {e:.3e}", flush=True)
             assert e <= 1e-6
-        elif shape[1] == 4096:
+        else:
             assert ran >= 1, f"{mode} did not run on the matrix cores"

@@ -80,8 +80,43 @@ def test_mfma_4096_vs_fp64(mode, tol, kind):
             assert errs[(mode, fwd)] > 5 * errs[("fp32", fwd)]
 
 
+@pytest.mark.parametrize("mode", ["mfma_fp16", "mfma_bf16", "mfma_bf16_2split"])
+@pytest.mark.parametrize("kind", ["normal", "loguniform"])
+def test_mfma_2p20_vs_fp64(mode, kind):
+    """complex64 lines of 2^20 = 1024 x 1024: both four-step passes run their
+    radix-16 stages on the matrix cores (one pass-A and one pass-B launch per
+    chunk of lines), both directions, against the long-double oracle."""
+    n = 1 << 20
+    rs = np.random.RandomState(5)
+    if kind == "normal":
+        x = rs.standard_normal((2, n)) + 1j * rs.standard_normal((2, n))
+    else:
+        x = np.exp(rs.uniform(np.log(1e-3), np.log(1e3), (2, n))) * \
+            np.exp(2j * np.pi * rs.uniform(size=(2, n)))
+    x = x.astype(np.complex64)
+    tol = 1e-4 if mode == "mfma_bf16_2split" else 1e-6
+    prev = R.set_precision(mode)
+    try:
+        for fwd, fct in ((True, 1.0), (False, 1.0 / n)):
+            t = torch.from_numpy(x).cuda()
+            y = torch.empty_like(t)
+            before = R.mfma_launches(mode)
+            R.c2c(t, y, np.array([1]), fwd, fct, 1)
+            torch.cuda.synchronize()
+            assert R.mfma_launches(mode) - before == 2, "both passes on the matrix cores"
+            want = oracle_out("c2c", x.astype(np.complex128), x.astype(np.complex128),
+                              np.array([1]), fwd, fct)
+            e = rel_l2(y.cpu().numpy(), want)
+            print(f"mode={mode} 2^20 input={kind} forward={fwd}: rel-L2 vs FP64 = {e:.3e}",
+                  flush=True)
+            assert e <= tol
+    finally:
+        R.set_precision(prev)
+
+
 def test_mfma_mode_only_touches_4096():
-    """Other lengths keep the plain path whatever the mode."""
+    """Other lengths keep the plain path whatever the mode (the contiguous
+    length-1024 lines here are not four-step passes)."""
     rs = np.random.RandomState(3)
     x = (rs.standard_normal((4, 1024)) + 1j * rs.standard_normal((4, 1024))).astype(np.complex64)
     prev = R.set_precision("mfma_bf16_2split")
