@@ -424,9 +424,32 @@ def ragged_config(torch, R, stream):
     worst = max(float(torch.linalg.norm(r - j[4]) / torch.linalg.norm(r)) for r, j in zip(ref, jobs))
     info = b.info()
     b.close()
+    # second comparison point: the jobs grouped by (kind, length), each group
+    # stacked into one array and transformed by one batched call
+    groups = {}
+    for kind, fwd, n, x, y, fct in jobs:
+        groups.setdefault((kind, n), []).append(x)
+    batched = []
+    for (kind, n), xs in groups.items():
+        X = torch.stack(xs)
+        Y = torch.empty((len(xs), n if kind == "c2c" else n // 2 + 1), dtype=torch.complex64,
+                        device="cuda")
+        batched.append((kind, X, Y))
+    ax = np.array([1])
+
+    def grouped():
+        for kind, X, Y in batched:
+            (R.c2c if kind == "c2c" else R.r2c)(X, Y, ax, True, 1.0, 1)
+
+    with torch.cuda.stream(stream):
+        grouped()
+    torch.cuda.synchronize()
+    grouped_ms, _ = graph_time(torch, grouped, stream, 3, batches=3)
     return {"jobs": len(jobs), "bytes": nbytes, "persistent_ms": round(ms, 4),
             "GB/s": round(nbytes / ms / 1e6, 1), "hbm_frac": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 3),
             "per_transform_ms": round(serial_ms, 3), "speedup_vs_per_transform": round(serial_ms / ms, 2),
+            "grouped_batched_ms": round(grouped_ms, 3), "distinct_kind_length_groups": len(batched),
+            "speedup_vs_grouped_batched": round(grouped_ms / ms, 2),
             "max_rel_l2_persistent_vs_per_transform": worst, "autotune": info}
 
 

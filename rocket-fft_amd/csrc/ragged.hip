@@ -1,17 +1,20 @@
 // ragged.hip -- many independent transforms of uneven lengths and kinds
-// (complex c2c, real r2c / c2r) in one persistent launch.
+// (complex c2c, real r2c / c2r) in persistent launches.
 //
 // A batch is planned once (rfh_batch_create): every job gets the passes the
 // per-call engine would run for it (direct LDS line, Bluestein in LDS, the
 // four-step passes, or pre -> c2c -> post for real lengths without a fused
 // plan), all sized to one LDS budget, with its temporaries carved out of one
-// arena.  Execution is two kernels: a fill that zeroes the job queue and one
-// persistent kernel whose workgroups pull whole jobs from that queue (largest
-// first) and run every pass of their job themselves, tile after tile, with
-// workgroup barriers between passes.  No workgroup ever waits on another,
-// so the launch always drains; each job's data stays on one CU and its
-// XCD's L2.  The LDS budget and the grid size are picked by a small
-// autotuner that times the candidates on the batch itself.
+// arena.  The passes are cut into work units (a run of tiles of one pass)
+// and grouped by level: level k holds the k-th pass of every job that has
+// one, so a unit depends only on units of earlier levels.  Execution is one
+// fill kernel that zeroes the levels' unit counters, then one persistent
+// launch per level (a job has at most a handful of passes): resident
+// workgroups pull units from that level's counter, largest job first, and
+// run their tiles.  Kernel boundaries order the levels, so no workgroup ever
+// waits on another and a large job's tiles spread over many CUs.  The LDS
+// budget and the grid size are picked by a small autotuner that times the
+// candidates on scratch copies of the batch's buffers.
 //
 // Execution allocates nothing and calls no host synchronisation, so it can
 // be captured in a HIP graph.
@@ -40,14 +43,17 @@ using namespace dev;
 
 constexpr int OP_PRE = 9;   // one line: real / half-complex input -> complex work line
 constexpr int OP_POST = 10; // one line: complex work line -> real / half-complex output
+// elements of work a unit gathers from consecutive tiles of one pass
+constexpr int64_t kUnitElems = 8192;
 
-struct JobDev {
-  int first;  // first pass
-  int npass;
-};
 struct PassDev {
   int op;
   int tiles;
+};
+struct UnitDev {  // tiles tile0 .. tile0 + ntiles - 1 of pass `pass`
+  int pass;
+  int tile0;
+  int ntiles;
 };
 
 template <typename T>
@@ -81,41 +87,39 @@ __device__ void post_line(const LineArgs<T> &a) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_ragged(const JobDev *__restrict__ jobs,
-                                                    const int *__restrict__ order, int njobs,
+__global__ void __launch_bounds__(kThreads) k_ragged(const UnitDev *__restrict__ units,
+                                                    int nunits,
                                                     const PassDev *__restrict__ passes,
                                                     const LineArgs<T> *__restrict__ args,
                                                     unsigned *queue) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int64_t s_oi[kMaxLinesPerGroup], s_oo[kMaxLinesPerGroup];
   __shared__ int s_aux[kMaxLinesPerGroup];
-  __shared__ int s_job;
+  __shared__ int s_unit;
   for (;;) {
-    if (threadIdx.x == 0) s_job = int(atomicAdd(queue, 1u));
+    if (threadIdx.x == 0) s_unit = int(atomicAdd(queue, 1u));
     __syncthreads();
-    const int q = __builtin_amdgcn_readfirstlane(s_job);
-    __syncthreads();  // every lane has read s_job before the next grab
-    if (q >= njobs) break;
-    const JobDev jb = jobs[order[q]];
-    for (int p = 0; p < jb.npass; ++p) {
-      const PassDev pd = passes[jb.first + p];
-      const LineArgs<T> &a = args[jb.first + p];
-      for (int t = 0; t < pd.tiles; ++t) {
-        switch (pd.op) {
-          case OP_C2C: line_body<T, OP_C2C, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
-          case OP_PASSA: line_body<T, OP_PASSA, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
-          case OP_PASSB: line_body<T, OP_PASSB, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
-          case OP_BLUE: line_body<T, OP_BLUE, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
-          case OP_BLUEROW: line_body<T, OP_BLUEROW, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
-          case OP_R2C_EVEN: line_body<T, OP_R2C_EVEN, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
-          case OP_R2C_ODD: line_body<T, OP_R2C_ODD, false, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
-          case OP_C2R_EVEN: line_body<T, OP_C2R_EVEN, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
-          case OP_C2R_ODD: line_body<T, OP_C2R_ODD, true, false>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
-          case OP_PRE: pre_line<T>(a); break;
-          default: post_line<T>(a); break;
-        }
-        __syncthreads();  // pass and tile boundary: global writes visible to the workgroup
+    const int q = __builtin_amdgcn_readfirstlane(s_unit);
+    __syncthreads();  // every lane has read s_unit before the next grab
+    if (q >= nunits) break;
+    const UnitDev u = units[q];
+    const PassDev pd = passes[u.pass];
+    const LineArgs<T> &a = args[u.pass];
+    for (int t = u.tile0; t < u.tile0 + u.ntiles; ++t) {
+      switch (pd.op) {
+        case OP_C2C: line_body<T, OP_C2C, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+        case OP_PASSA: line_body<T, OP_PASSA, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+        case OP_PASSB: line_body<T, OP_PASSB, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+        case OP_BLUE: line_body<T, OP_BLUE, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+        case OP_BLUEROW: line_body<T, OP_BLUEROW, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+        case OP_R2C_EVEN: line_body<T, OP_R2C_EVEN, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+        case OP_R2C_ODD: line_body<T, OP_R2C_ODD, false, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+        case OP_C2R_EVEN: line_body<T, OP_C2R_EVEN, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+        case OP_C2R_ODD: line_body<T, OP_C2R_ODD, true, false>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
+        case OP_PRE: pre_line<T>(a); break;
+        default: post_line<T>(a); break;
       }
+      __syncthreads();  // tile boundary: LDS and per-line slots are reused
     }
   }
 }
@@ -128,7 +132,8 @@ __global__ void k_zero(unsigned *p, int n) {
 
 struct Built {
   size_t budget = 0;          // dynamic LDS per workgroup
-  DevBuf jobs, order, passes, args, queue, arena;
+  DevBuf units, passes, args, queue, arena;
+  std::vector<int> level_off;  // units of level k: [level_off[k], level_off[k + 1])
   std::vector<std::shared_ptr<C2CPlan>> c2c_keep;
   std::vector<std::shared_ptr<RealPlan>> real_keep;
   int njobs = 0;
@@ -244,15 +249,15 @@ std::unique_ptr<Built> build(const std::vector<rfh_job_t> &jobs, bool dbl, size_
     post.sub = (post_mode << 8) | (r2c && !j.forward ? R2R_CONJ : 0);
     per[i].push_back(PassLaunch{post, one_line(work, j.out, 2, r2c ? 2 : 1)});
   }
-  // pass 3: device tables
-  std::vector<JobDev> jd(jobs.size());
+  // pass 3: device tables; units by level, the jobs of a level largest first
   std::vector<PassDev> pd;
   std::vector<LineArgs<T>> la;
   std::vector<std::pair<double, int>> cost;
-  int lds_need = 0;
+  std::vector<int> first(jobs.size());
+  int lds_need = 0, levels = 0;
   for (size_t i = 0; i < jobs.size(); ++i) {
-    jd[i].first = int(pd.size());
-    jd[i].npass = int(per[i].size());
+    first[i] = int(pd.size());
+    levels = std::max(levels, int(per[i].size()));
     for (const PassLaunch &pl : per[i]) {
       if (pl.ps.op == OP_BLUE_IN || pl.ps.op == OP_BLUE_MUL || pl.ps.op == OP_BLUE_OUT)
         RFH_THROW(3, "ragged batch: length " + std::to_string(jobs[i].n) +
@@ -276,14 +281,27 @@ std::unique_ptr<Built> build(const std::vector<rfh_job_t> &jobs, bool dbl, size_
                    [](const std::pair<double, int> &x, const std::pair<double, int> &y) {
                      return x.first > y.first;
                    });
-  std::vector<int> order;
-  for (auto &c : cost) order.push_back(c.second);
-  B->jobs.upload(jd.data(), jd.size() * sizeof(JobDev));
-  B->order.upload(order.data(), order.size() * sizeof(int));
+  std::vector<UnitDev> units;
+  B->level_off.assign(1, 0);
+  for (int lv = 0; lv < levels; ++lv) {
+    for (const auto &c : cost) {
+      const size_t i = size_t(c.second);
+      if (int(per[i].size()) <= lv) continue;
+      const int pid = first[i] + lv;
+      const LineArgs<T> &a = la[size_t(pid)];
+      // a unit holds enough tiles for about kUnitElems elements of work
+      const int64_t per_tile = int64_t(std::max(1, a.L)) * std::max(1, a.f.n);
+      const int tpu = int(std::max<int64_t>(1, kUnitElems / per_tile));
+      for (int t0 = 0; t0 < pd[size_t(pid)].tiles; t0 += tpu)
+        units.push_back(UnitDev{pid, t0, std::min(tpu, pd[size_t(pid)].tiles - t0)});
+    }
+    B->level_off.push_back(int(units.size()));
+  }
+  B->units.upload(units.data(), units.size() * sizeof(UnitDev));
   B->passes.upload(pd.data(), pd.size() * sizeof(PassDev));
   B->args.upload(la.data(), la.size() * sizeof(LineArgs<T>));
-  unsigned zero = 0;
-  B->queue.upload(&zero, sizeof(zero));
+  std::vector<unsigned> zeros(size_t(std::max(levels, 1)), 0u);
+  B->queue.upload(zeros.data(), zeros.size() * sizeof(unsigned));
   for (auto &p : cplan)
     if (p) B->c2c_keep.push_back(p);
   for (auto &p : rplan)
@@ -310,14 +328,20 @@ void launch_batch(const Built &B, int grid, hipStream_t s) {
       g_attr_set.insert({dev, int(sizeof(T) == 8)});
     }
   }
-  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, static_cast<unsigned *>(B.queue.p), 1);
-  hipLaunchKernelGGL(k_ragged<T>, dim3(unsigned(grid)), dim3(kThreads), size_t(B.lds_need), s,
-                     static_cast<const JobDev *>(B.jobs.p), static_cast<const int *>(B.order.p),
-                     B.njobs, static_cast<const PassDev *>(B.passes.p),
-                     static_cast<const LineArgs<T> *>(B.args.p),
-                     static_cast<unsigned *>(B.queue.p));
-  const hipError_t e = hipGetLastError();
-  if (e != hipSuccess) RFH_THROW(2, std::string("ragged launch: ") + hipGetErrorString(e));
+  const int levels = int(B.level_off.size()) - 1;
+  unsigned *queue = static_cast<unsigned *>(B.queue.p);
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, queue, levels);
+  for (int lv = 0; lv < levels; ++lv) {
+    const int n = B.level_off[size_t(lv) + 1] - B.level_off[size_t(lv)];
+    if (n == 0) continue;
+    hipLaunchKernelGGL(k_ragged<T>, dim3(unsigned(std::min(grid, n))), dim3(kThreads),
+                       size_t(B.lds_need), s,
+                       static_cast<const UnitDev *>(B.units.p) + B.level_off[size_t(lv)], n,
+                       static_cast<const PassDev *>(B.passes.p),
+                       static_cast<const LineArgs<T> *>(B.args.p), queue + lv);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) RFH_THROW(2, std::string("ragged launch: ") + hipGetErrorString(e));
+  }
 }
 
 int cu_count() {
