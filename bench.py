@@ -453,9 +453,34 @@ def ragged_config(torch, R, stream):
             "max_rel_l2_persistent_vs_per_transform": worst, "autotune": info}
 
 
+def probe_ragged(torch, R):
+    """The 4096-job ragged workload: planned (no autotune) and executed three
+    times, no graph (trace and counter runs)."""
+    spec = R.ragged_workload(njobs=4096, seed=5)
+    jobs = []
+    for kind, n in spec:
+        if kind == "c2c":
+            x = torch.randn(n, dtype=torch.complex64, device="cuda")
+            y = torch.empty(n, dtype=torch.complex64, device="cuda")
+        else:
+            x = torch.randn(n, dtype=torch.float32, device="cuda")
+            y = torch.empty(n // 2 + 1, dtype=torch.complex64, device="cuda")
+        jobs.append((kind, True, n, x, y, 1.0))
+    b = R.RaggedBatch(jobs, double=False, autotune=False)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.execute()
+    torch.cuda.synchronize()
+    b.close()
+    print("probe done", flush=True)
+
+
 def probe(torch, R, shape, mode):
     """Calibration copy (reads and writes the array once each, three times),
     then three transforms of `shape` in `mode`, no graph (counter runs)."""
+    if shape == "ragged":
+        return probe_ragged(torch, R)
+    shape = tuple(int(v) for v in shape.split("x"))
     x = torch.randn(shape, dtype=torch.complex64, device="cuda")
     y = torch.empty_like(x)
     torch.cuda.synchronize()
@@ -480,7 +505,8 @@ def main():
                     help="arithmetic of the step (default: the fastest form within 1e-6)")
     ap.add_argument("--probe", default=None, metavar="SHAPE:MODE",
                     help="only a device copy and three transforms of one shape "
-                         "('64x1048576:fp32', '16384x4096:mfma_fp16', ...), for counter runs")
+                         "('64x1048576:fp32', '16384x4096:mfma_fp16', 'ragged:fp32', ...), "
+                         "for counter runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: every transform of this workload fits one "
@@ -493,7 +519,7 @@ def main():
 
     if args.probe:
         shp, mode = args.probe.split(":")
-        probe(torch, R, tuple(int(v) for v in shp.split("x")), mode)
+        probe(torch, R, shp, mode)
         return
     stream = torch.cuda.Stream()
     data = {shape: precision_sets(torch, shape) for shape in SHAPES}

@@ -122,6 +122,51 @@ int rfh_set_fast_path(bool enabled);
 int rfh_kernel_timing(bool enabled);
 int rfh_kernel_timing_report(char *buf, size_t len);
 
+/* ---- shifted transforms -----------------------------------------------------
+ * numba_c2c with numpy.roll shifts fused into the transform:
+ *   aout = roll(c2c(roll(ain, shift_in, axes)), shift_out, axes),
+ * shift_in[i] / shift_out[i] (any sign; NULL: none) rolling along axes[i],
+ * each axis at most once.  fftshift is a roll by n // 2, ifftshift by
+ * -(n // 2) (overloads.py:1253-1310 compose them with the transforms as
+ * separate calls).  The rolls become index rotations in the loads of each
+ * axis's first pass and the stores of its last pass, so no extra pass runs
+ * and the result equals the composition bit for bit.  Fusable: lengths whose
+ * plan is one LDS pass, and four-step lengths n1 x n2 whose input roll is a
+ * multiple of n2 and output roll a multiple of n1 (every fftshift /
+ * ifftshift of an even length with both factors even); other plans
+ * (Bluestein, multi-pass) return 5 with nothing written, and the caller
+ * runs the rolls and the transform separately.  Complex64 transforms run
+ * on the FP32 kernels whatever rfh_set_precision says.  No reference
+ * counterpart.  Returns 0, 5 (not fusable), or another error code. */
+int rfh_c2c_shifted(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout, arystruct_t *axes,
+                    bool forward, double fct, uint64_t nthreads, const int64_t *shift_in,
+                    const int64_t *shift_out);
+
+/* ---- plan cache -------------------------------------------------------------
+ * Plans (radix schedules, FP64-generated twiddle tables, the Bluestein
+ * kernel's spectrum) are built once per (device, length, precision, LDS
+ * budget, contiguous / interleaved lines) and kept for the process, as the
+ * reference keeps its plans in pocketfft's LRU cache
+ * (_pocketfft_hdronly.h:3170-3222, get_plan).  rfh_plan_cache adds an
+ * on-disk cache of the host records under `dir`, keyed by transform kind,
+ * length, precision, LDS budget, stride class and GPU architecture, so that a
+ * later process skips the table generation; dir NULL or "" turns it off.
+ * Files are checksummed and written atomically; one that does not check is
+ * ignored and rewritten.  clear_memory also drops this process's plans from
+ * the cache (the next transform of each shape rebuilds or reloads its plan;
+ * the dropped plans' device tables stay allocated until the process ends, so
+ * work already enqueued and graphs already captured stay valid).  No
+ * reference counterpart for the disk part.  Returns 0 or an error code. */
+int rfh_plan_cache(const char *dir, bool clear_memory);
+/* Plans loaded from / written to the disk cache since the library loaded. */
+int rfh_plan_cache_stats(uint64_t *hits, uint64_t *writes);
+/* Host-only (no GPU needed), for tests of the file format: builds the plan
+ * record of (n, precision, lds_budget > 0 bytes, interleaved, real), writes
+ * it to `path`, reads it back and compares.  0: identical; 1: the file did
+ * not read back; 2: a field differs; -1: error (rfh_last_error). */
+int rfh_plan_record_roundtrip(uint64_t n, bool double_precision, uint64_t lds_budget,
+                              bool interleaved, bool real, const char *path);
+
 /* ---- allocation-free variant --------------------------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, serving loops, harnesses that forbid

@@ -373,7 +373,14 @@ struct LineArgs {
   int64_t other; // four-step: N2 in pass A, N1 in pass B
   int shift;
   const C2<T> *t0, *t1, *t2;  // op tables
+  int rot_in, rot_out;  // element-index rotations of the line set (0: none)
 };
+
+// (j + r) mod n for 0 <= j, r < n
+__device__ __forceinline__ int rot_index(int j, int r, int n) {
+  const int q = j + r;
+  return q >= n ? q - n : q;
+}
 
 template <bool V, typename T>
 __device__ __forceinline__ C2<T> ldc(const T *p) {
@@ -823,7 +830,9 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
   auto load = [&](int l, int j) -> C2<T> {
     const T *base = a.in + s_oi[l];
     C2<T> v;
-    if constexpr (OP == OP_C2C || OP == OP_PASSB || OP == OP_BLUEROW) {
+    if constexpr (OP == OP_C2C) {
+      v = ldc<VIN>(base + rot_index(j, a.rot_in, a.nin) * a.esi);
+    } else if constexpr (OP == OP_PASSB || OP == OP_BLUEROW) {
       v = ldc<VIN>(base + j * a.esi);
     } else if constexpr (OP == OP_PASSA) {
       if (a.sub & SUB_CHIRP) {
@@ -835,7 +844,7 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
           v = C2<T>{T(0), T(0)};
         }
       } else {
-        v = ldc<VIN>(base + j * a.esi);
+        v = ldc<VIN>(base + rot_index(j, a.rot_in, a.nin) * a.esi);
       }
     } else if constexpr (OP == OP_BLUE) {
       if (j < a.bign) {
@@ -889,7 +898,7 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
     if (k >= a.nout) return;
     T *base = a.out + s_oo[l];
     if constexpr (OP == OP_C2C) {
-      stc<VOUT>(base + k * a.eso, a.fct * v);
+      stc<VOUT>(base + rot_index(k, a.rot_out, a.nout) * a.eso, a.fct * v);
     } else if constexpr (OP == OP_PASSA) {
       if (a.sub & SUB_FINAL) {  // Bluestein columns, last pass: chirp and fct, m < n
         const int64_t m = s_aux[l] + int64_t(k) * a.other;
@@ -915,7 +924,7 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
           stc<VOUT>(base + k * a.eso, a.fct * cmul(v, conjf ? c : conjc(c)));
         }
       } else {
-        stc<VOUT>(base + k * a.eso, a.fct * v);
+        stc<VOUT>(base + rot_index(k, a.rot_out, a.nout) * a.eso, a.fct * v);
       }
     } else if constexpr (OP == OP_BLUE) {
       const C2<T> c = a.t0[k];

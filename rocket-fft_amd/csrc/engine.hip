@@ -63,7 +63,9 @@ void fork_guard() {
 void set_error(int code, const std::string &msg) {
   t_err = code;
   t_msg = msg;
-  std::fprintf(stderr, "rocket_fft_hip: %s\n", msg.c_str());
+  // code 5 (a shift that cannot be fused) is an answer for the caller, who
+  // then runs the unfused form, not a failure
+  if (code != 5) std::fprintf(stderr, "rocket_fft_hip: %s\n", msg.c_str());
 }
 
 // -------------------------------------------------------------- workspace
@@ -215,19 +217,31 @@ struct Ctx {
   size_t scalar;
 };
 
-void c2c_axis(Ctx &c, const int64_t *shape, int nd, int ax, const int64_t *si,
-              const int64_t *so, const void *in, void *out, bool forward, double fct) {
+struct AxisPlan {
+  LineSet g;
+  std::shared_ptr<C2CPlan> plan;
+};
+
+AxisPlan c2c_axis_plan(Ctx &c, const int64_t *shape, int nd, int ax, const int64_t *si,
+                       const int64_t *so, const void *in, void *out) {
   const uint64_t n = uint64_t(shape[ax]);
-  LineSet g = lines_for_axis(shape, nd, ax, si, so, in, out);
+  AxisPlan r;
+  r.g = lines_for_axis(shape, nd, ax, si, so, in, out);
   // lines side by side in memory (a batch dim closer than the elements of
   // a line, on either side) want the interleaved plan
   bool inter = false;
-  for (int d = 0; d < g.nb; ++d)
-    inter = inter || std::llabs(g.bsi[d]) < std::llabs(g.esi) ||
-            std::llabs(g.bso[d]) < std::llabs(g.eso);
-  auto plan = get_c2c_plan(n, c.dbl, 0, inter);
-  void *tmp = c.ws->get(kSlotTwoPass, c2c_tmp_bytes(*plan, g));
-  launch_c2c(*plan, g, forward ? -1 : +1, fct, c.s, tmp);
+  for (int d = 0; d < r.g.nb; ++d)
+    inter = inter || std::llabs(r.g.bsi[d]) < std::llabs(r.g.esi) ||
+            std::llabs(r.g.bso[d]) < std::llabs(r.g.eso);
+  r.plan = get_c2c_plan(n, c.dbl, 0, inter);
+  return r;
+}
+
+void c2c_axis(Ctx &c, const int64_t *shape, int nd, int ax, const int64_t *si,
+              const int64_t *so, const void *in, void *out, bool forward, double fct) {
+  AxisPlan ap = c2c_axis_plan(c, shape, nd, ax, si, so, in, out);
+  void *tmp = c.ws->get(kSlotTwoPass, c2c_tmp_bytes(*ap.plan, ap.g));
+  launch_c2c(*ap.plan, ap.g, forward ? -1 : +1, fct, c.s, tmp);
 }
 
 void c2c_nd(Ctx &c, const int64_t *shape, int nd, const int64_t *si, const int64_t *so,
@@ -490,6 +504,52 @@ void c2c(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout, arystruct_t *
     c.dbl = dbl;
     c2c_nd(c, in.shape, in.ndim, in.str, out.str, ax.data(), int(ax.size()), in.data,
            out.data, forward, fct);
+  });
+}
+
+// c2c with numpy.roll shifts fused into the loads of each axis's first pass
+// and the stores of its last pass: aout = roll(c2c(roll(ain, shift_in)),
+// shift_out), the rolls along axes[i] by shift_in[i] / shift_out[i].
+void c2c_shifted(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout, arystruct_t *axes,
+                 bool forward, double fct, const int64_t *shift_in, const int64_t *shift_out) {
+  const bool dbl = ain->itemsize == 16;
+  const size_t sc = dbl ? 8 : 4;
+  const int64_t *shape = ain->shape_and_strides;
+  if (any_zero(shape, int(ndim))) return;
+  std::vector<uint64_t> ax;
+  check_axes(axes, ndim, ax);
+  if (ax.empty()) return;
+  for (size_t i = 0; i < ax.size(); ++i)
+    for (size_t j = 0; j < i; ++j)
+      if (ax[i] == ax[j]) RFH_THROW(3, "a shifted transform takes each axis once");
+  std::vector<int64_t> rin(ax.size()), rout(ax.size());
+  for (size_t i = 0; i < ax.size(); ++i) {
+    const int64_t n = shape[ax[i]];
+    const int64_t a = shift_in ? shift_in[i] % n : 0, b = shift_out ? shift_out[i] % n : 0;
+    // element j of the rolled input is input element j - shift_in
+    rin[i] = a <= 0 ? -a : n - a;
+    rout[i] = b < 0 ? b + n : b;
+  }
+  Arr in = view(ndim, ain, shape, sc), out = view(ndim, aout, aout->shape_and_strides, sc);
+  check_fits(out, shape, "output");
+  for (int d = 0; d < out.ndim; ++d) out.shape[d] = shape[d];
+  with_device_arrays(in, 2, out, 2, sc, [&](Ctx &c) {
+    c.dbl = dbl;
+    std::vector<AxisPlan> aps;
+    for (size_t i = 0; i < ax.size(); ++i) {
+      aps.push_back(c2c_axis_plan(c, in.shape, in.ndim, int(ax[i]), i == 0 ? in.str : out.str,
+                                  out.str, i == 0 ? in.data : out.data, out.data));
+      if (!c2c_rotation_fusable(*aps.back().plan, rin[i], rout[i]))
+        RFH_THROW(5, "the shift along axis " + std::to_string(ax[i]) +
+                         " cannot be fused into this length's passes");
+    }
+    for (size_t i = 0; i < aps.size(); ++i) {
+      AxisPlan &ap = aps[i];
+      ap.g.rot_in = rin[i];
+      ap.g.rot_out = rout[i];
+      void *tmp = c.ws->get(kSlotTwoPass, c2c_tmp_bytes(*ap.plan, ap.g));
+      launch_c2c(*ap.plan, ap.g, forward ? -1 : +1, i == 0 ? fct : 1.0, c.s, tmp);
+    }
   });
 }
 
@@ -880,6 +940,34 @@ int rfh_kernel_timing_report(char *buf, size_t len) {
 
 int rfh_mfma_operands(int mode, int direction, float *out, int cap) {
   return mfma_operand_table(mode, direction, out, cap);
+}
+
+int rfh_c2c_shifted(uint64_t ndim, const arystruct_t *ain, arystruct_t *aout, arystruct_t *axes,
+                    bool forward, double fct, uint64_t nthreads, const int64_t *shift_in,
+                    const int64_t *shift_out) {
+  (void)nthreads;
+  guarded([&] { ops::c2c_shifted(ndim, ain, aout, axes, forward, fct, shift_in, shift_out); });
+  return t_err;
+}
+
+int rfh_plan_cache(const char *dir, bool clear_memory) {
+  guarded([&] { plan_cache_dir(dir ? std::string(dir) : std::string(), clear_memory); });
+  return t_err;
+}
+
+int rfh_plan_cache_stats(uint64_t *hits, uint64_t *writes) {
+  plan_cache_stats(hits, writes);
+  return 0;
+}
+
+int rfh_plan_record_roundtrip(uint64_t n, bool double_precision, uint64_t lds_budget,
+                              bool interleaved, bool real, const char *path) {
+  int rc = -1;
+  if (!lds_budget || !path) return -1;
+  guarded([&] {
+    rc = plan_record_roundtrip(n, double_precision, size_t(lds_budget), interleaved, real, path);
+  });
+  return t_err ? -1 : rc;
 }
 
 int rfh_set_fast_path(bool enabled) {

@@ -109,9 +109,9 @@ __device__ __forceinline__ C2<T> fload(const FCtx<T> &c, int p) {
       const C2<T> ch = a.t2[m];
       return cmul(ldc<VIN>(c.gin + p * a.esi), (a.sub & SUB_CONJ) ? ch : conjc(ch));
     }
-    return ldc<VIN>(c.gin + p * a.esi);
-  } else {
-    return ldc<VIN>(c.gin + p * a.esi);
+    return ldc<VIN>(c.gin + ((p + a.rot_in) & (a.f.n - 1)) * a.esi);
+  } else {  // OP_C2C (power-of-two lengths: the rotation is a mask)
+    return ldc<VIN>(c.gin + ((p + a.rot_in) & (a.f.n - 1)) * a.esi);
   }
 }
 
@@ -131,7 +131,7 @@ template <typename T, int OP, bool VOUT>
 __device__ __forceinline__ void fstore(const LineArgs<T> &a, const C2<T> *twl, T *gout, int aux,
                                        int p, C2<T> v) {
   if constexpr (OP == OP_C2C) {
-    stc<VOUT>(gout + p * a.eso, a.fct * v);
+    stc<VOUT>(gout + ((p + a.rot_out) & (a.f.n - 1)) * a.eso, a.fct * v);
   } else if constexpr (OP == OP_PASSA) {
     if (a.sub & SUB_FINAL) {  // Bluestein columns, last pass: chirp and fct, m < n
       const int64_t m = aux + int64_t(p) * a.other;
@@ -155,7 +155,7 @@ __device__ __forceinline__ void fstore(const LineArgs<T> &a, const C2<T> *twl, T
         stc<VOUT>(gout + p * a.eso, a.fct * cmul(v, cj ? ch : conjc(ch)));
       }
     } else {
-      stc<VOUT>(gout + p * a.eso, a.fct * v);
+      stc<VOUT>(gout + ((p + a.rot_out) & (a.f.n - 1)) * a.eso, a.fct * v);
     }
   } else if constexpr (OP == OP_C2R_EVEN) {
     const C2<T> z = a.fct * v;
@@ -539,7 +539,8 @@ __global__ void __launch_bounds__(TPL *LPB, SPLIT ? 4 : 1) k_fast(const LineArgs
     for (int f = t; f < nl * NIN; f += TPL * LPB) {
       const int l2 = f / NIN, p = f - l2 * NIN;
       const T *g = a.in + s_oi[l2];
-      lds[l2 * P + padi<T>(p)] = ldc<true>(g + (OP == OP_R2C_EVEN ? 2 * p : p * a.esi));
+      const int q = OP == OP_C2C ? ((p + a.rot_in) & (N - 1)) : p;
+      lds[l2 * P + padi<T>(p)] = ldc<true>(g + (OP == OP_R2C_EVEN ? 2 * p : q * a.esi));
     }
     __syncthreads();
   }

@@ -173,6 +173,8 @@ LineArgs<T> make_line_args(const PassSpec &ps, const LineSet &g) {
   a.t0 = static_cast<const C2<T> *>(ps.t0);
   a.t1 = static_cast<const C2<T> *>(ps.t1);
   a.t2 = static_cast<const C2<T> *>(ps.t2);
+  a.rot_in = int(g.rot_in);
+  a.rot_out = int(g.rot_out);
   return a;
 }
 
@@ -619,6 +621,16 @@ size_t c2c_tmp_bytes(const C2CPlan &p, const LineSet &g, size_t chunk_bytes) {
   }
 }
 
+bool c2c_rotation_fusable(const C2CPlan &p, int64_t rot_in, int64_t rot_out) {
+  if (rot_in < 0 || rot_out < 0 || uint64_t(rot_in) >= p.n || uint64_t(rot_out) >= p.n)
+    return false;
+  if (!rot_in && !rot_out) return true;
+  if (p.kind == kDirect) return true;
+  if (p.kind == kTwoPass)
+    return uint64_t(rot_in) % p.n2 == 0 && uint64_t(rot_out) % p.n1 == 0;
+  return false;
+}
+
 std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct,
                                      void *tmp, size_t chunk_bytes) {
   const int64_t cl = chunk_lines(p, g, chunk_bytes);
@@ -637,6 +649,8 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
   const int sub_conj = dir > 0 ? SUB_CONJ : 0;
   const int lines = dbl ? 4 : 8;
   const bool inter = input_interleaved(g);
+  if ((g.rot_in || g.rot_out) && !c2c_rotation_fusable(p, g.rot_in, g.rot_out))
+    RFH_THROW(3, "this plan cannot fuse the requested rotation into its passes");
   if (p.kind == kDirect) {
     PassSpec ps;
     ps.d = &p.main;
@@ -671,7 +685,13 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     a.t1 = p.big_hi.p;
     a.nin = a.nout = int(n1);
     a.lines = pass_lines(p, n1, lines);
-    out.push_back(PassLaunch{a, pass_a(g, n1, n2, g.in, g.esi, g.bsi, t)});
+    // rotations by whole rows / columns: input element j1 n2 + c of a line is
+    // column c's element j1, so rot_in = t n2 rotates each column by t;
+    // output bin k1 + n1 k2 is row k1's bin k2, so rot_out = u n1 rotates
+    // each row by u
+    LineSet la = pass_a(g, n1, n2, g.in, g.esi, g.bsi, t);
+    la.rot_in = g.rot_in / int64_t(n2);
+    out.push_back(PassLaunch{a, la});
     PassSpec b;
     b.d = &p.pb;
     b.op = OP_PASSB;
@@ -680,7 +700,9 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     b.other = int64_t(n1);
     b.nin = b.nout = int(n2);
     b.lines = pass_lines(p, n2, lines);
-    out.push_back(PassLaunch{b, pass_b(g, n1, n2, t, g.out, g.eso, g.bso)});
+    LineSet lb = pass_b(g, n1, n2, t, g.out, g.eso, g.bso);
+    lb.rot_out = g.rot_out / int64_t(n1);
+    out.push_back(PassLaunch{b, lb});
   } else if (p.kind == kMultiPass) {
     // columns of n1 with the twiddle of N, then the sub-plan of M on the rows
     const uint64_t n1 = p.n1, M = p.n2;

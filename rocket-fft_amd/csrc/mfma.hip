@@ -619,6 +619,7 @@ bool launch_pass1024(int mode, int op, const dev::LineArgs<float> &a, bool lf_in
 bool launch_mfma(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out, bool vin,
                  bool vout, hipStream_t s) {
   if (!g_precision_mode || !vin || !vout) return false;
+  if (a.rot_in || a.rot_out) return false;  // rotated lines run on the FP32 kernels
   if (a.f.n == 1024 && (op == OP_PASSA || op == OP_PASSB)) {
     if (!launch_pass1024(g_precision_mode, op, a, lf_in, lf_out, s)) return false;
     g_mfma_launches[g_precision_mode].fetch_add(1);

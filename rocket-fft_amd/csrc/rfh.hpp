@@ -139,6 +139,15 @@ std::shared_ptr<RealPlan> get_real_plan(uint64_t n, bool dbl, size_t lds_budget 
 // null when none was built yet (never allocates).
 std::shared_ptr<C2CPlan> find_c2c_plan(uint64_t n, bool dbl, size_t lds_budget = 0,
                                        bool interleaved = false);
+// On-disk cache of plan records under `dir` (empty: off, the default);
+// `clear_memory` also drops the plans built so far in this process.
+void plan_cache_dir(const std::string &dir, bool clear_memory);
+// Plans read from / written to the disk cache so far.
+void plan_cache_stats(uint64_t *hits, uint64_t *writes);
+// Host-only: a plan record written to `path` and read back; 0 when every
+// field survives, 1 when the file does not read back, 2 when it differs.
+int plan_record_roundtrip(uint64_t n, bool dbl, size_t budget, bool inter, bool real,
+                          const std::string &path);
 
 // Radix schedule (execution order) for an LDS-resident FFT of length n.
 std::vector<int> gpu_radices(uint64_t n);
@@ -168,6 +177,12 @@ struct LineSet {
   int64_t bso[kMaxBatchDims] = {};
   int64_t esi = 0, eso = 0;  // element strides (scalar units)
   int auxdim = -1;           // batch dim whose index the four-step ops use (-1: nb - 1)
+  // circular rotations of the element index (0 <= rot < line length): the
+  // transform's input element j is read from position (j + rot_in) mod n of
+  // its line and its bin k is written to position (k + rot_out) mod n, which
+  // fuses numpy.roll / fftshift / ifftshift along the axis into the
+  // transform's loads and stores
+  int64_t rot_in = 0, rot_out = 0;
 };
 
 // Kernel launchers (kernels.hip).  All enqueue on `s`.
@@ -178,6 +193,11 @@ struct LineSet {
 void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStream_t s,
                 void *tmp, size_t chunk_bytes = 0);
 size_t c2c_tmp_bytes(const C2CPlan &p, const LineSet &g, size_t chunk_bytes = 0);
+// Whether plan p runs a line set with element rotations rot_in / rot_out
+// (0 <= rot < n) inside its own passes: one-pass plans always; four-step
+// plans when rot_in is a whole number of columns (n2) and rot_out of rows
+// (n1); the other kinds only without rotations.
+bool c2c_rotation_fusable(const C2CPlan &p, int64_t rot_in, int64_t rot_out);
 
 // Opt-in per-launch timing of the line kernels (HIP events on the launching
 // stream); the report waits for the recorded launches and lists, one line per

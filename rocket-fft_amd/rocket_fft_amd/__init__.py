@@ -222,6 +222,59 @@ def c2c(ain, aout, axes, forward, fct, nthreads=1):
     return _call("numba_c2c", ain, aout, axes, f, s, t)
 
 
+NOT_FUSABLE = 5
+
+
+def c2c_shifted(ain, aout, axes, forward, fct, shift_in=None, shift_out=None, nthreads=1):
+    """c2c with numpy.roll shifts fused into the transform (rfh_c2c_shifted):
+    aout = roll(c2c(roll(ain, shift_in, axes)), shift_out, axes), shift_in[i]
+    and shift_out[i] (None: no roll) along axes[i], each axis once.  Returns
+    True when the rolls were fused; False, with nothing written, when a
+    length's plan cannot fuse them (the caller then rolls separately)."""
+    f, s, t = _cmplx_args(forward, fct, nthreads)
+    S_in, nd, kin, cin = _struct(ain)
+    S_out, nd2, kout, cout = _struct(aout)
+    if nd != nd2:
+        raise TypeError("Input and output array must have the same number of dimensions")
+    S_ax, keep = _axes_struct(axes, nd)
+    nax = len(np.atleast_1d(np.asarray(axes)))
+
+    def shifts(v):
+        if v is None:
+            return None
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.int64), (nax,)))
+        return a
+
+    si, so = shifts(shift_in), shifts(shift_out)
+    L = _load()
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    L.rfh_c2c_shifted.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_bool, ctypes.c_double,
+                                  ctypes.c_uint64, i64p, i64p]
+    L.rfh_c2c_shifted.restype = ctypes.c_int
+    stream = None
+    if cin or cout:
+        import torch
+        stream = torch.cuda.current_stream(aout.device if cout else ain.device).cuda_stream
+        L.rfh_set_stream(ctypes.c_void_p(stream), False)
+    try:
+        L.rfh_clear_error()
+        rc = L.rfh_c2c_shifted(nd, ctypes.byref(S_in), ctypes.byref(S_out), ctypes.byref(S_ax),
+                               f, s, t,
+                               None if si is None else si.ctypes.data_as(i64p),
+                               None if so is None else so.ctypes.data_as(i64p))
+        code, msg = last_error()
+    finally:
+        if stream is not None:
+            L.rfh_set_stream(None, True)
+    del keep
+    if rc == NOT_FUSABLE:
+        return False
+    if rc or code:
+        raise RuntimeError(f"rfh_c2c_shifted: {msg}")
+    return True
+
+
 def c2c_noalloc(ain, aout, axes, forward, fct):
     """c2c that allocates no device memory (rfh_c2c_noalloc): device tensors
     only, enqueued on torch's current stream without synchronising; its
@@ -355,6 +408,44 @@ def set_precision(mode):
     L.rfh_set_precision.restype = ctypes.c_int
     prev = L.rfh_set_precision(PRECISION_MODES[mode])
     return {v: k for k, v in PRECISION_MODES.items()}[prev]
+
+
+def plan_cache(directory, clear_memory=False):
+    """Keep plan records (twiddle and Bluestein tables) on disk under
+    `directory` (None: off), keyed by kind, length, precision, LDS budget,
+    stride class and GPU architecture (rfh_plan_cache); `clear_memory` also
+    drops the plans built so far in this process."""
+    L = _load()
+    L.rfh_plan_cache.argtypes = [ctypes.c_char_p, ctypes.c_bool]
+    L.rfh_plan_cache.restype = ctypes.c_int
+    d = b"" if directory is None else os.fsencode(directory)
+    if L.rfh_plan_cache(d, bool(clear_memory)) != 0:
+        raise RuntimeError(last_error()[1])
+
+
+def plan_cache_stats():
+    """(plans loaded from disk, plans written to disk) so far."""
+    L = _load()
+    L.rfh_plan_cache_stats.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.rfh_plan_cache_stats.restype = ctypes.c_int
+    hits, writes = ctypes.c_uint64(), ctypes.c_uint64()
+    L.rfh_plan_cache_stats(ctypes.byref(hits), ctypes.byref(writes))
+    return int(hits.value), int(writes.value)
+
+
+def plan_record_roundtrip(n, double_precision, lds_budget, interleaved=False, real=False,
+                          path=None):
+    """Host-only check of the plan file format: 0 when the record of the
+    plan survives a write to `path` and a read back (rfh_plan_record_roundtrip)."""
+    L = _load()
+    L.rfh_plan_record_roundtrip.argtypes = [ctypes.c_uint64, ctypes.c_bool, ctypes.c_uint64,
+                                            ctypes.c_bool, ctypes.c_bool, ctypes.c_char_p]
+    L.rfh_plan_record_roundtrip.restype = ctypes.c_int
+    rc = L.rfh_plan_record_roundtrip(int(n), bool(double_precision), int(lds_budget),
+                                     bool(interleaved), bool(real), os.fsencode(path))
+    if rc < 0:
+        raise RuntimeError(last_error()[1])
+    return rc
 
 
 def kernel_timing(enabled):

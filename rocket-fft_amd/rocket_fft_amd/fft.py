@@ -27,14 +27,14 @@ import warnings
 
 import numpy as np
 
-from . import (c2c, c2c_sym, c2r, dct as _dct_ll, dst as _dst_ll, good_size, line_scale, r2c,
-               roll_device)
+from . import (c2c, c2c_shifted, c2c_sym, c2r, dct as _dct_ll, dst as _dst_ll, good_size,
+               line_scale, r2c, roll_device)
 
 __all__ = [
     "fft", "ifft", "fft2", "ifft2", "fftn", "ifftn", "rfft", "irfft", "rfft2", "irfft2",
     "rfftn", "irfftn", "hfft", "ihfft", "fftfreq", "rfftfreq", "fftshift", "ifftshift",
     "dct", "idct", "dst", "idst", "dctn", "idctn", "dstn", "idstn", "next_fast_len",
-    "fht", "ifht", "fhtoffset", "roll",
+    "fht", "ifht", "fhtoffset", "roll", "fftn_shifted", "centered_fftn", "centered_ifftn",
 ]
 
 
@@ -348,6 +348,70 @@ def fftshift(x, axes=None):
 
 def ifftshift(x, axes=None):
     return _shift(_asarray(x), axes, -1)
+
+
+# ------------------------------------------- transforms with fused shifts
+
+def _shift_amounts(spec, shape, axes):
+    """Roll amounts along axes for a shift spec: None, "fftshift" (n // 2),
+    "ifftshift" (-(n // 2)) or integers (one, or one per axis)."""
+    if spec is None:
+        return None
+    if isinstance(spec, str):
+        if spec == "fftshift":
+            return [shape[ax] // 2 for ax in axes]
+        if spec == "ifftshift":
+            return [-(shape[ax] // 2) for ax in axes]
+        raise ValueError("shift must be None, 'fftshift', 'ifftshift' or integers")
+    v = np.broadcast_to(np.asarray(spec, dtype=np.int64), (len(axes),))
+    return [int(x) for x in v]
+
+
+def fftn_shifted(a, s=None, axes=None, norm=None, forward=True, shift_in=None,
+                 shift_out=None):
+    """roll(fftn(roll(a, shift_in), s, axes, norm), shift_out) (ifftn when
+    forward is False), the rolls along the transformed axes.  A shift is None,
+    "fftshift", "ifftshift" or integer roll amounts.  numpy composes these as
+    separate calls (overloads.py:1253-1310 for the shifts); here the rolls
+    become index rotations in the loads of each axis's first pass and the
+    stores of its last pass (rfh_c2c_shifted), with a bit-identical result.
+    Lengths whose plan cannot fuse them (Bluestein, multi-pass) fall back to
+    one separate gather pass on each side."""
+    a = _asarray(a)
+    s, axes = _shape_and_axes(a, s, axes)
+    rdt, cdt = _dtype_pair(a)
+    x = _pad_crop(a, s, axes, cdt)
+    si = _shift_amounts(shift_in, x.shape, axes)
+    so = _shift_amounts(shift_out, x.shape, axes)
+    fct = _fct(x.shape, axes, norm, forward)
+    out = _empty(x, x.shape, cdt)
+    if x.ndim == 0 or min(x.shape) == 0:
+        return out
+    if c2c_shifted(x, out, _ax(axes), forward, fct, si, so, 1):
+        return out
+    if si is not None:
+        shifts = [0] * x.ndim
+        for ax, v in zip(axes, si):
+            shifts[ax] += v
+        x = _roll_gpu(x, shifts)
+    c2c(x, out, _ax(axes), forward, fct, 1)
+    if so is not None:
+        shifts = [0] * x.ndim
+        for ax, v in zip(axes, so):
+            shifts[ax] += v
+        out = _roll_gpu(out, shifts)
+    return out
+
+
+def centered_fftn(a, s=None, axes=None, norm=None):
+    """fftshift(fftn(ifftshift(a))) over axes, the DFT of data centred on
+    index n // 2, in one fused transform."""
+    return fftn_shifted(a, s, axes, norm, True, "ifftshift", "fftshift")
+
+
+def centered_ifftn(a, s=None, axes=None, norm=None):
+    """fftshift(ifftn(ifftshift(a))) over axes, the inverse of centered_fftn."""
+    return fftn_shifted(a, s, axes, norm, False, "ifftshift", "fftshift")
 
 
 # ------------------------------------------------------------ scipy.fft
