@@ -40,11 +40,17 @@ SHAPES = ((64, 1 << 20), (16384, 4096))
 BYTES_PER_SHAPE = 2 * 64 * (1 << 20) * 8
 BYTES_STEP = 2 * BYTES_PER_SHAPE
 MODES = ("fp32", "mfma_fp16", "mfma_bf16")
-# HBM bytes per launch of the roofline kernels, measured with
-#   rocprofv3 --pmc FETCH_SIZE -- python3 bench.py --probe   (then WRITE_SIZE)
-# each counter scaled by what it reads for a device copy of known size in the
-# same run (copy-calibrated); None where not measured for this kernel.
-TRAFFIC = {}
+# HBM bytes per launch of the step's kernels, measured with
+#   rocprofv3 --pmc FETCH_SIZE -- python3 bench.py --probe 64x1048576:fp32
+#   (then WRITE_SIZE; and the same for 16384x4096:fp32)
+# each counter scaled by what it reads for the probe's device copies of the
+# same array in the same run (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000);
+# None where not measured for a kernel.
+TRAFFIC = {
+    "k_fast passA n=1024 c64 strided": 269552478,  # 1.004 x the 268,435,456 algorithmic
+    "k_fast passB n=1024 c64": 268849614,          # 1.002 x
+    "k_fast c2c n=4096 c64": 1074157236,           # 1.000 x 1,073,741,824
+}
 
 
 def rel_l2(a, b):
