@@ -1023,6 +1023,11 @@ bool launch_fast(int op, const dev::LineArgs<T> &a, bool lf_in, bool lf_out, boo
                  bool vout, hipStream_t s);
 extern bool g_fast_enabled;
 
+// Persistent software-pipelined FP32 passes of the complex64 1024 x 1024
+// four-step plans (pass1024.hip); false when the pass does not match.
+bool launch_pass1024(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out, bool vin,
+                     bool vout, hipStream_t s);
+
 // Split-precision MFMA kernels (mfma.hip); false when the mode is off or the
 // pass is not a contiguous complex64 line of 4096.
 bool launch_mfma(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out, bool vin,

@@ -71,6 +71,7 @@ struct FCtx {
   int aux;
   bool valid;
   int i;           // index of the thread within its line
+  T dir;           // sign of the stages being run (-1 forward)
 };
 
 // c2r: input p of the half-length complex transform from bins p and h - p
@@ -143,6 +144,9 @@ __device__ __forceinline__ void fstore(const LineArgs<T> &a, const C2<T> *twl, T
       const C2<T> w = dirw(ftwiddle(a, twl, int64_t(p) * aux), a.dir);
       stc<VOUT>(gout + p * a.eso, cmul(v, w));
     }
+  } else if constexpr (OP == OP_BLUEROW) {  // inverse four-step twiddle, in place
+    const C2<T> w = dirw(big_twiddle(a, int64_t(p) * aux), T(1));
+    stc<VOUT>(gout + p * a.eso, cmul(v, w));
   } else if constexpr (OP == OP_PASSB) {
     const int64_t m = aux + int64_t(p) * a.other;
     const bool cj = (a.sub & SUB_CONJ) != 0;
@@ -260,7 +264,7 @@ __device__ __forceinline__ void fstage(const FCtx<T> &c) {
   constexpr bool FIRST = S == 0;
   constexpr bool LAST = S == SC::count - 1;
   const LineArgs<T> &a = *c.a;
-  const T dir = a.dir;
+  const T dir = c.dir;
   C2<T> v[Q][R];
   static_assert(!SHUF || (FIRST && Q == 1 && NR == TPL && OP == OP_C2R_EVEN), "shuffled c2r load");
 #pragma unroll
@@ -379,7 +383,7 @@ __device__ __forceinline__ void fstage_split(
   constexpr bool LAST = S == SC::count - 1;
   static_assert(OP != OP_R2C_EVEN, "r2c reads two bins per output: full exchange only");
   const LineArgs<T> &a = *c.a;
-  const T dir = a.dir;
+  const T dir = c.dir;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int j = c.i + q * TPL;
@@ -496,6 +500,7 @@ __global__ void __launch_bounds__(TPL *LPB, SPLIT ? 4 : 1) k_fast(const LineArgs
   const int64_t b = tile * LPB + l;
   FCtx<T> c;
   c.a = &a;
+  c.dir = a.dir;
   c.twl = nullptr;
   if constexpr (OP == OP_PASSA) {
     if (a.twlds_lo > 0) {  // copy the four-step twiddle tables behind the lines
@@ -530,6 +535,24 @@ __global__ void __launch_bounds__(TPL *LPB, SPLIT ? 4 : 1) k_fast(const LineArgs
       s_oo[l] = oo;
       s_aux[l] = aux;
     }
+  }
+  if constexpr (OP == OP_BLUEROW) {
+    // Bluestein rows: forward FFT (global -> natural order in LDS), times
+    // the kernel t2 at m = aux + k other, inverse FFT out of LDS, stored in
+    // place with the inverse four-step twiddle; every load of a line comes
+    // before the barriers that precede its stores
+    fstage<T, N, TPL, OP, VIN, VOUT, false, SC, 0, false>(c);
+    if (c.valid) {
+      const bool cj = (a.sub & SUB_CONJ) != 0;
+      for (int k = i; k < N; k += TPL) {
+        const C2<T> w = a.t2[aux + int64_t(k) * a.other];
+        c.line[padi<T>(k)] = cmul(c.line[padi<T>(k)], cj ? conjc(w) : w);
+      }
+    }
+    __syncthreads();
+    c.dir = -a.dir;
+    fstage<T, N, TPL, OP, VIN, VOUT, true, SC, 0, true>(c);
+    return;
   }
   if constexpr (FLAT) {
     if (i == 0) s_oi[l] = oi;
@@ -699,6 +722,15 @@ bool go_len(const LineArgs<T> &a, bool lf_in, bool lf_out, bool vin, bool vout,
             hipStream_t s) {
   using F = FastShape<T, N>;
   constexpr bool r2c = OP == OP_R2C_EVEN;
+  if constexpr (OP == OP_BLUEROW) {  // in place: one layout, the full exchange
+    if (lf_in != lf_out) return false;
+    if (lf_in) {
+      if constexpr (F::interleaved_ok)
+        return go_map<T, N, F::TPL, F::LPB_I, OP, SC, false>(a, lf_in, lf_out, vin, vout, s);
+      return false;
+    }
+    return go_map<T, N, F::TPL, F::LPB_C, OP, SC, false>(a, lf_in, lf_out, vin, vout, s);
+  }
   if (lf_in || lf_out) {
     if constexpr (F::interleaved_ok)
       return go_map<T, N, F::TPL, F::LPB_I, OP, SC, F::SPLIT_I && !r2c>(a, lf_in, lf_out, vin,
@@ -765,6 +797,7 @@ bool launch_fast(int op, const LineArgs<T> &a, bool lf_in, bool lf_out, bool vin
     case OP_C2C: return go_op<T, OP_C2C>(a, lf_in, lf_out, vin, vout, s);
     case OP_PASSA: return go_op<T, OP_PASSA>(a, lf_in, lf_out, vin, vout, s);
     case OP_PASSB: return go_op<T, OP_PASSB>(a, lf_in, lf_out, vin, vout, s);
+    case OP_BLUEROW: return go_op<T, OP_BLUEROW>(a, lf_in, lf_out, vin, vout, s);
     case OP_R2C_EVEN: return go_op<T, OP_R2C_EVEN>(a, lf_in, lf_out, vin, vout, s);
     case OP_C2R_EVEN: return go_op<T, OP_C2R_EVEN>(a, lf_in, lf_out, vin, vout, s);
     default: return false;

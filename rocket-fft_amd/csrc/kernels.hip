@@ -322,11 +322,16 @@ void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
     vout = g.eso == 1 && pairs_aligned(g.out, sizeof(T), 0, g.bso, g.nb);
   else
     vout = pairs_aligned(g.out, sizeof(T), g.eso, g.bso, g.nb);
-  if constexpr (sizeof(T) == 4)
+  if constexpr (sizeof(T) == 4) {
     if (launch_mfma(ps.op, a, a.lfast_in, a.lfast_out, vin, vout, s)) {
       t_last_family = "mfma";
       return;
     }
+    if (launch_pass1024(ps.op, a, a.lfast_in, a.lfast_out, vin, vout, s)) {
+      t_last_family = "k_pass1024";
+      return;
+    }
+  }
   if (launch_fast<T>(ps.op, a, a.lfast_in, a.lfast_out, vin, vout, s)) {
     t_last_family = "k_fast";
     return;

@@ -14,8 +14,8 @@ PKG_ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(PKG_ROOT, "csrc")
 BUILD = os.path.join(PKG_ROOT, "_build")
 LIB = os.path.join(HERE, "librocketfft_hip.so")
-SOURCES = ["plan.hip", "kernels.hip", "fast.hip", "mfma.hip", "ragged.hip", "engine.hip",
-           "lineops.hip"]
+SOURCES = ["plan.hip", "kernels.hip", "fast.hip", "mfma.hip", "pass1024.hip", "ragged.hip",
+           "engine.hip", "lineops.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",

@@ -76,6 +76,52 @@ def test_bluestein_two_pass_pow2_columns(dtype):
     check("c2r", xc, xr, np.array([1]), False, 1.0 / n)
 
 
+@pytest.mark.parametrize("lines", [3, 17])
+def test_two_pass_2p20_pipelined(lines):
+    """complex64 2^20 = 1024 x 1024: both four-step passes run on the
+    persistent pipelined kernel (checked through the per-launch timing
+    labels); 17 lines make a chunk of 16 and a chunk of 1 (64 tiles, fewer
+    than the CUs).  Both directions against the oracle."""
+    import torch
+    n = 1 << 20
+    rs = np.random.RandomState(lines)
+    x = crandn(rs, (lines, n), np.complex64)
+    R.kernel_timing_report()
+    R.kernel_timing(True)
+    try:
+        check("c2c", x, x, np.array([1]), True, 1.0)
+    finally:
+        torch.cuda.synchronize()
+        R.kernel_timing(False)
+    labels = R.kernel_timing_report()
+    print("labels", sorted(labels), flush=True)
+    assert any(k.startswith("k_pass1024 passA n=1024") for k in labels)
+    assert any(k.startswith("k_pass1024 passB n=1024") for k in labels)
+    check("c2c", x, x, np.array([1]), False, 1.0 / n)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("n", [8193, 32749, 65537])
+def test_bluestein_pow2_rows(dtype, n):
+    """Bluestein lengths whose padding from 2n - 2 is a power of two (8193 and
+    65537 = 2^k + 1 pad to exactly 2^(k+1); prime 32749 to 65536) split into
+    power-of-two columns and rows, so both the column passes and the row pass
+    (forward FFT, kernel product, inverse FFT, inverse twiddle, in place) run
+    on the specialised kernel.  Contiguous and strided (axis 0) lines, both
+    directions."""
+    dbl = dtype == np.complex128
+    kind, _, n1, n2 = R.gpu_schedule(n, dbl)
+    print(f"case bluestein pow2 rows n={n} {dtype.__name__} schedule={kind, n1, n2}", flush=True)
+    assert kind == 3 and n1 * n2 >= 2 * n - 2 and n2 & (n2 - 1) == 0
+    rs = np.random.RandomState(n)
+    x = crandn(rs, (3, n), dtype)
+    check("c2c", x, x, np.array([1]), True, 1.0)
+    check("c2c", x, x, np.array([1]), False, 1.0 / n)
+    xt = crandn(rs, (n, 5), dtype)
+    check("c2c", xt, xt, np.array([0]), True, 1.0)
+    check("c2c", xt, xt, np.array([0]), False, 0.5, inplace=True)
+
+
 @pytest.mark.parametrize("shape,axes", [((16, 24, 10), (0, 1, 2)), ((16, 24, 10), (2, 0)),
                                         ((33, 17), (0,)), ((33, 17), (1, 0)),
                                         ((5, 6, 7, 8), (3, 1)), ((64, 3), (0,))])
