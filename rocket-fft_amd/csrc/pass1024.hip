@@ -29,6 +29,7 @@ namespace {
 using namespace dev;
 
 constexpr int kN = 1024, kL = 16, kP = kN + kN / 16 + 1;  // padded line pitch
+constexpr int kSt2 = 256;  // stage-2 twiddles kept in LDS
 typedef __attribute__((ext_vector_type(2))) unsigned u2;
 
 __device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
@@ -91,7 +92,20 @@ __global__ void __launch_bounds__(1024, 1)
       s_aux[s][t] = o.aux;
     }
   };
-  C2<float> *twl = reinterpret_cast<C2<float> *>(smem_raw + size_t(kL) * kP * sizeof(C2<float>));
+  // behind the lines: stage 2's twiddles W_1024^j (j < 256) with the
+  // direction applied, whose squares and cubes give its r = 2, 3 factors,
+  // then (pass A, when the launch reserved room) the four-step twiddle's
+  // lo / hi tables.  Stage 1 reads its 240 twiddles from global memory
+  // (cached): in LDS the compiler issues them with the line reads and runs
+  // out of registers.
+  const float dir = a.dir;
+  C2<float> *st2l = lds + kL * kP;
+  C2<float> *twl = st2l + kSt2;
+  const C2<float> *tw1 = a.tw + a.f.st[1].twoff;
+  {
+    const C2<float> *tw2 = a.tw + a.f.st[2].twoff;
+    for (int q = t; q < kSt2; q += 1024) st2l[q] = dirw(tw2[q], dir);
+  }
   const bool tw_lds = PASS_A && a.twlds_lo > 0;
   if (tw_lds) {
     const int nt = a.twlds_lo + a.twlds_hi;
@@ -100,9 +114,6 @@ __global__ void __launch_bounds__(1024, 1)
   const __amdgpu_buffer_rsrc_t rin = make_rsrc(a.in, in_bytes);
   const __amdgpu_buffer_rsrc_t rout = make_rsrc(a.out, out_bytes);
   const uint32_t es_in = uint32_t(a.esi) * 4u, es_out = uint32_t(a.eso) * 4u;
-  const float dir = a.dir;
-  const C2<float> *tw1 = a.tw + a.f.st[1].twoff;
-  const C2<float> *tw2 = a.tw + a.f.st[2].twoff;
   fill(tile_of(0), 0);
   if (tile_of(1) < ntiles) fill(tile_of(1), 1);
   __syncthreads();
@@ -127,6 +138,10 @@ __global__ void __launch_bounds__(1024, 1)
   load_tile(0, tile_of(0));
 
   for (int64_t k = 0;; ++k) {
+    // an opaque zero in the stage-twiddle indices keeps their reads in the
+    // loop: hoisted out, they would hold 30+ registers the pipeline needs
+    int z;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
     const int s = int(k % 3);
     const int64_t tile = tile_of(k);
     // ---- stage 0 (radix 16, NS 1) from registers into LDS
@@ -150,7 +165,7 @@ __global__ void __launch_bounds__(1024, 1)
 #pragma unroll
       for (int r = 0; r < 16; ++r) w[r] = ln[pad(j + 64 * r)];
 #pragma unroll
-      for (int r = 1; r < 16; ++r) w[r] = cmul(w[r], dirw(tw1[(r - 1) * 16 + kk], dir));
+      for (int r = 1; r < 16; ++r) w[r] = cmul(w[r], dirw(tw1[(r - 1) * 16 + kk + z], dir));
       dft<16>(w, dir, static_cast<const C2<float> *>(nullptr));
       __syncthreads();  // every read of stage 1 is done
 #pragma unroll
@@ -159,10 +174,6 @@ __global__ void __launch_bounds__(1024, 1)
     __syncthreads();
     // ---- stage 2 (radix 4, NS 256), natural-order bins to global memory
     if (tile * kL + l2 < a.nlines) {
-      // an opaque zero keeps the stage-2 twiddle loads inside the loop: held
-      // across tiles, they would cost 24 registers the pipeline needs
-      int zero;
-      asm volatile("s_mov_b32 %0, 0" : "=s"(zero));
       const C2<float> *ln = lds + l2 * kP;
       const uint32_t vo = uint32_t(s_oo[s][l2]) * 4u + uint32_t(i2) * es_out;
       const int aux = s_aux[s][l2];
@@ -172,8 +183,10 @@ __global__ void __launch_bounds__(1024, 1)
         C2<float> u[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) u[r] = ln[pad(j + 256 * r)];
-#pragma unroll
-        for (int r = 1; r < 4; ++r) u[r] = cmul(u[r], dirw(tw2[(r - 1) * 256 + j + zero], dir));
+        const C2<float> w1 = st2l[j + z], w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+        u[1] = cmul(u[1], w1);
+        u[2] = cmul(u[2], w2);
+        u[3] = cmul(u[3], w3);
         dft4(u, dir);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -257,7 +270,7 @@ bool launch_pass1024(int op, const LineArgs<float> &a0, bool lf_in, bool lf_out,
   LineArgs<float> a = a0;
   const int64_t ntiles = (a.nlines + kL - 1) / kL;
   if (ntiles <= 0) return true;
-  size_t lds = size_t(kL) * kP * sizeof(C2<float>);
+  size_t lds = size_t(kL * kP + kSt2) * sizeof(C2<float>);
   a.twlds_lo = a.twlds_hi = 0;
   if (pass_a) {  // four-step twiddle tables in LDS when they fit
     const int64_t big = int64_t(kN) * a.other;
