@@ -29,7 +29,7 @@ namespace {
 using namespace dev;
 
 constexpr int kN = 1024, kL = 16, kP = kN + kN / 16 + 1;  // padded line pitch
-constexpr int kSt2 = 256;  // stage-2 twiddles kept in LDS
+constexpr int kSt1 = 16 * 16, kSt2 = 256;  // stage twiddles kept in LDS
 typedef __attribute__((ext_vector_type(2))) unsigned u2;
 
 __device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
@@ -92,22 +92,23 @@ __global__ void __launch_bounds__(1024, 1)
       s_aux[s][t] = o.aux;
     }
   };
-  // behind the lines: stage 2's twiddles W_1024^j (j < 256) with the
-  // direction applied, whose squares and cubes give its r = 2, 3 factors,
-  // then (pass A, when the launch reserved room) the four-step twiddle's
-  // lo / hi tables.  Stage 1 reads its 240 twiddles from global memory
-  // (cached): in LDS the compiler issues them with the line reads and runs
-  // out of registers.
+  // behind the lines, with the direction applied: stage 1's twiddles
+  // W_256^(r k) ([r 16 + k], r, k < 16), stage 2's W_1024^j (j < 256, whose
+  // squares and cubes give its r = 2, 3 factors), then (pass A) the
+  // four-step twiddle's lo / hi tables.  No global load runs between a
+  // prefetch and its use: the memory counter completes in order, so waiting
+  // for any later load would also wait for the whole prefetch.
   const float dir = a.dir;
-  C2<float> *st2l = lds + kL * kP;
+  C2<float> *st1l = lds + kL * kP;
+  C2<float> *st2l = st1l + kSt1;
   C2<float> *twl = st2l + kSt2;
-  const C2<float> *tw1 = a.tw + a.f.st[1].twoff;
   {
+    const C2<float> *tw1 = a.tw + a.f.st[1].twoff;
     const C2<float> *tw2 = a.tw + a.f.st[2].twoff;
-    for (int q = t; q < kSt2; q += 1024) st2l[q] = dirw(tw2[q], dir);
+    for (int q = t; q < kSt1 + kSt2; q += 1024)
+      st1l[q] = q < 16 ? C2<float>{1.f, 0.f} : dirw(q < kSt1 ? tw1[q - 16] : tw2[q - kSt1], dir);
   }
-  const bool tw_lds = PASS_A && a.twlds_lo > 0;
-  if (tw_lds) {
+  if (PASS_A) {
     const int nt = a.twlds_lo + a.twlds_hi;
     for (int q = t; q < nt; q += 1024) twl[q] = q < a.twlds_lo ? a.t0[q] : a.t1[q - a.twlds_lo];
   }
@@ -144,12 +145,18 @@ __global__ void __launch_bounds__(1024, 1)
     asm volatile("s_mov_b32 %0, 0" : "=s"(z));
     const int s = int(k % 3);
     const int64_t tile = tile_of(k);
-    // ---- stage 0 (radix 16, NS 1) from registers into LDS
+    // ---- stage 0 (radix 16, NS 1) from registers into LDS.  Output r of
+    // butterfly i0 lands at position 16 i0 + r = (16 (i0 mod 4) + r) +
+    // 64 (i0 / 4): stage 1's input i0 / 4 of its butterfly 16 (i0 mod 4) + r,
+    // whose twiddle W_256^((i0 / 4) r) is applied here, while no prefetch
+    // holds registers
     dft<16>(v, dir, static_cast<const C2<float> *>(nullptr));
     {
       C2<float> *ln = lds + l0 * kP;
+      const C2<float> *tw = st1l + (i0 >> 2) * 16 + z;  // row 0: ones
+      ln[pad(i0 * 16)] = v[0];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) ln[pad(i0 * 16 + r)] = v[r];
+      for (int r = 1; r < 16; ++r) ln[pad(i0 * 16 + r)] = cmul(v[r], tw[r]);
     }
     // offsets of tile k + 2 (read after this iteration's barriers)
     if (tile_of(k + 2) < ntiles) fill(tile_of(k + 2), int((k + 2) % 3));
@@ -163,9 +170,7 @@ __global__ void __launch_bounds__(1024, 1)
       C2<float> *ln = lds + l0 * kP;
       const int j = i0, kk = j & 15;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) w[r] = ln[pad(j + 64 * r)];
-#pragma unroll
-      for (int r = 1; r < 16; ++r) w[r] = cmul(w[r], dirw(tw1[(r - 1) * 16 + kk + z], dir));
+      for (int r = 0; r < 16; ++r) w[r] = ln[pad(j + 64 * r)];  // twiddled in stage 0
       dft<16>(w, dir, static_cast<const C2<float> *>(nullptr));
       __syncthreads();  // every read of stage 1 is done
 #pragma unroll
@@ -177,7 +182,7 @@ __global__ void __launch_bounds__(1024, 1)
       const C2<float> *ln = lds + l2 * kP;
       const uint32_t vo = uint32_t(s_oo[s][l2]) * 4u + uint32_t(i2) * es_out;
       const int aux = s_aux[s][l2];
-#pragma unroll
+#pragma unroll 1  // unrolled, the four groups hold more registers than are left
       for (int q = 0; q < 4; ++q) {
         const int j = i2 + 64 * q;
         C2<float> u[4];
@@ -192,15 +197,10 @@ __global__ void __launch_bounds__(1024, 1)
         for (int r = 0; r < 4; ++r) {
           const int kb = j + 256 * r;
           C2<float> x;
-          if constexpr (PASS_A) {
-            const int64_t e = int64_t(kb) * aux;
-            C2<float> w;
-            if (tw_lds) {
-              const int64_t mask = (int64_t(1) << a.shift) - 1;
-              w = cmul(twl[e & mask], twl[a.twlds_lo + (e >> a.shift)]);
-            } else {
-              w = big_twiddle(a, e);
-            }
+          if constexpr (PASS_A) {  // W_N^e from the LDS tables, e < 2^31
+            const unsigned e = unsigned(kb) * unsigned(aux);
+            const unsigned mask = (1u << a.shift) - 1u;
+            const C2<float> w = cmul(twl[e & mask], twl[a.twlds_lo + int(e >> a.shift)]);
             x = cmul(u[r], dirw(w, dir));
           } else {
             x = a.fct * u[r];
@@ -270,17 +270,16 @@ bool launch_pass1024(int op, const LineArgs<float> &a0, bool lf_in, bool lf_out,
   LineArgs<float> a = a0;
   const int64_t ntiles = (a.nlines + kL - 1) / kL;
   if (ntiles <= 0) return true;
-  size_t lds = size_t(kL * kP + kSt2) * sizeof(C2<float>);
+  size_t lds = size_t(kL * kP + kSt1 + kSt2) * sizeof(C2<float>);
   a.twlds_lo = a.twlds_hi = 0;
-  if (pass_a) {  // four-step twiddle tables in LDS when they fit
+  if (pass_a) {  // the four-step twiddle tables go to LDS; no room: decline
     const int64_t big = int64_t(kN) * a.other;
     const int64_t lo_n = int64_t(1) << a.shift, hi_n = (big + lo_n - 1) / lo_n;
     const size_t extra = size_t(lo_n + hi_n) * sizeof(C2<float>);
-    if (lds + extra + 2048 <= lds_limit()) {
-      a.twlds_lo = int(lo_n);
-      a.twlds_hi = int(hi_n);
-      lds += extra;
-    }
+    if (big >= (int64_t(1) << 31) || lds + extra + 2048 > lds_limit()) return false;
+    a.twlds_lo = int(lo_n);
+    a.twlds_hi = int(hi_n);
+    lds += extra;
   }
   const void *fn =
       pass_a ? (rot ? reinterpret_cast<const void *>(&k_pass1024<true, true>)
