@@ -54,6 +54,19 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t voff, 
   __builtin_amdgcn_raw_buffer_store_b64(v, r, int(voff), int(soff), 0);
 }
 
+// The exchanges of stages 0 and 1 stay inside a line.  In pass B a wave owns
+// its line there, and a wave's LDS operations complete in issue order, so
+// only the compiler must keep them in order; pass A's lines span waves.
+template <bool PASS_A>
+__device__ __forceinline__ void line_sync() {
+  if constexpr (PASS_A) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+  }
+}
+
 // in_bytes / out_bytes: spans of the line set from a.in / a.out (all
 // offsets non-negative, below 2^31); ntiles = ceil(nlines / 16).  ROT: the
 // line set carries element rotations (pass A loads, pass B stores), whose
@@ -163,7 +176,7 @@ __global__ void __launch_bounds__(1024, 1)
     // loads of tile k + 1 go out now and land during stages 1 and 2
     const bool more = tile_of(k + 1) < ntiles;
     if (more) load_tile(int((k + 1) % 3), tile_of(k + 1));
-    __syncthreads();
+    line_sync<PASS_A>();
     // ---- stage 1 (radix 16, NS 16)
     {
       C2<float> w[16];
@@ -172,7 +185,7 @@ __global__ void __launch_bounds__(1024, 1)
 #pragma unroll
       for (int r = 0; r < 16; ++r) w[r] = ln[pad(j + 64 * r)];  // twiddled in stage 0
       dft<16>(w, dir, static_cast<const C2<float> *>(nullptr));
-      __syncthreads();  // every read of stage 1 is done
+      line_sync<PASS_A>();  // every read of stage 1 is done
 #pragma unroll
       for (int r = 0; r < 16; ++r) ln[pad((j - kk) * 16 + kk + r * 16)] = w[r];
     }

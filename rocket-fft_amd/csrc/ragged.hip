@@ -420,7 +420,19 @@ DiskKey disk_key(const std::vector<uint64_t> &sig) {
   return DiskKey{h1, h2, uint64_t(sig.size())};
 }
 
-bool budget_ok(size_t b) { return b == 32768 || b == 65536 || b == std::min<size_t>(98304, lds_limit() - 8192); }
+// LDS budgets the autotuner tries (the device limit less the kernel's static
+// LDS caps the larger two)
+std::vector<size_t> tune_budgets() {
+  const size_t cap = lds_limit() - 8192;
+  return {size_t(32768), size_t(65536), std::min<size_t>(98304, cap),
+          std::min<size_t>(147456, cap)};
+}
+
+bool budget_ok(size_t b) {
+  for (size_t c : tune_budgets())
+    if (b == c) return true;
+  return false;
+}
 
 std::string cache_file() {
   if (g_cache_dir.empty()) return {};
@@ -533,11 +545,10 @@ void autotune(Batch &bt, hipStream_t s) {
       tb.jobs[i].out = base + off[i].second;
     }
   }
-  const size_t cap = lds_limit() - 8192;
   double best = 1e300;
   size_t best_budget = 0;
   int best_grid = 0;
-  for (size_t budget : {size_t(32768), size_t(65536), std::min<size_t>(98304, cap)}) {
+  for (size_t budget : tune_budgets()) {
     try {
       tb.b = build_any(tb, budget);
     } catch (const Error &) {
