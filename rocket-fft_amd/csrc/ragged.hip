@@ -44,7 +44,7 @@ using namespace dev;
 constexpr int OP_PRE = 9;   // one line: real / half-complex input -> complex work line
 constexpr int OP_POST = 10; // one line: complex work line -> real / half-complex output
 // elements of work a unit gathers from consecutive tiles of one pass
-constexpr int64_t kUnitElems = 8192;
+constexpr int64_t kUnitElems = 4096;
 
 struct PassDev {
   int op;
