@@ -25,7 +25,7 @@ def check(fname, ain, aout_like, *args, inplace=False, tol=None):
     return e
 
 
-C2C_LENGTHS = [1, 2, 3, 4, 5, 7, 8, 11, 13, 16, 17, 30, 47, 64, 97, 100, 128, 243, 256,
+C2C_LENGTHS = [1, 2, 3, 4, 5, 7, 8, 11, 13, 16, 17, 30, 32, 47, 64, 97, 100, 128, 243, 256,
                343, 512, 1000, 1024, 1536, 2187, 4096, 4913, 8192, 4099, 8232, 16384]
 
 
@@ -74,6 +74,18 @@ def test_bluestein_two_pass_pow2_columns(dtype):
     check("r2c", xr, yr, np.array([1]), True, 1.0)
     xc = crandn(rs, (2, n // 2 + 1), dtype)
     check("c2r", xc, xr, np.array([1]), False, 1.0 / n)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("n,lines", [(32, 1000), (64, 777), (32, 3), (64, 1)])
+def test_short_lines_partial_tiles(dtype, n, lines):
+    """Contiguous lines of 32 / 64 (2 / 4 lanes per line on the specialised
+    kernel): line counts that leave partial tiles, both directions, in
+    place."""
+    rs = np.random.RandomState(n + lines)
+    x = crandn(rs, (lines, n), dtype)
+    check("c2c", x, x, np.array([1]), True, 1.0)
+    check("c2c", x, x, np.array([1]), False, 1.0 / n, inplace=True)
 
 
 @pytest.mark.parametrize("lines", [3, 17])
