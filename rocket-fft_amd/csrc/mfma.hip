@@ -37,6 +37,7 @@
 #include <tuple>
 #include <vector>
 
+#include "bufio.hpp"
 #include "passes.hpp"
 
 namespace rfh {
@@ -472,6 +473,205 @@ __global__ void __launch_bounds__(1024, 1) k_pass1024_mfma(const LineArgs<float>
   }
 }
 
+// The same passes, persistent and software-pipelined like the FP32 form
+// (pass1024.hip): one 1024-thread workgroup per CU walks its 16-line tiles
+// and issues the next tile's loads (buffer instructions, scalar per-element
+// offsets) right after the current tile's first stage; no global load runs
+// between a prefetch and its use (the memory counter completes in order), so
+// stage 1's twiddles are applied in stage 0 from an LDS table, stage 2's come
+// from LDS, and pass A's four-step tables sit in LDS.  Pass B's stage 0 -> 1
+// exchange stays inside a wave (its line) and takes no workgroup barrier.
+constexpr int kPipeL = 16, kPipeP = 1024 + 64 + 1, kPipeSt1 = 256, kPipeSt2 = 256;
+
+template <int MODE, bool PASS_A>
+__global__ void __launch_bounds__(1024, 1)
+    k_pass1024_mfma_pipe(const LineArgs<float> a, const float *__restrict__ aop, int64_t ntiles,
+                         uint32_t in_bytes, uint32_t out_bytes) {
+  using O = Ops<MODE>;
+  typedef typename O::V V;
+  constexpr int L = kPipeL, P = kPipeP;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C2<float> *lds = reinterpret_cast<C2<float> *>(smem_raw);
+  __shared__ int64_t s_oi[3][L], s_oo[3][L];
+  __shared__ int s_aux[3][L];
+  __shared__ float s_max[16][L];
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63, col = lane & 15, kg = lane >> 4;
+  const int r0 = (kg & 1) * 8 + (kg >= 2 ? 4 : 0);
+  const int ml = PASS_A ? col : wave;  // line of this lane in the radix-16 stages
+  auto bj = [&](int g) { return PASS_A ? wave * 4 + g : g * 16 + col; };
+  const int l2 = t & 15, i2 = t >> 4;  // stage 2 and the stores
+  const int64_t G = gridDim.x, b = blockIdx.x;
+  const int64_t slot0 = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+  if (slot0 >= ntiles) return;
+  auto tile_of = [&](int64_t k) { return k * G + slot0; };
+  auto fill = [&](int64_t tile, int s) {
+    if (t < L) {
+      const LineOffsets o =
+          group_slot(tile * L, t, L, a.nlines, a.nb, a.bdim, a.bsi, a.bso, a.auxdim);
+      s_oi[s][t] = o.oi;
+      s_oo[s][t] = o.oo;
+      s_aux[s][t] = o.aux;
+    }
+  };
+  const float dir = a.dir;
+  // behind the lines: stage-1 twiddles ([r 16 + k] = W_256^(r k), row 0
+  // ones), stage 2's W_1024^j (j < 256), pass A's four-step lo / hi tables
+  C2<float> *st1l = lds + L * P;
+  C2<float> *st2l = st1l + kPipeSt1;
+  C2<float> *twl = st2l + kPipeSt2;
+  {
+    const C2<float> *tw1 = a.tw + a.f.st[1].twoff;
+    const C2<float> *tw2 = a.tw + a.f.st[2].twoff;
+    for (int q = t; q < kPipeSt1 + kPipeSt2; q += 1024)
+      st1l[q] = q < 16 ? C2<float>{1.f, 0.f}
+                       : dirw(q < kPipeSt1 ? tw1[q - 16] : tw2[q - kPipeSt1], dir);
+  }
+  if (PASS_A) {
+    const int nt = a.twlds_lo + a.twlds_hi;
+    for (int q = t; q < nt; q += 1024) twl[q] = q < a.twlds_lo ? a.t0[q] : a.t1[q - a.twlds_lo];
+  }
+  V Are[O::terms], Aim[O::terms];
+#pragma unroll
+  for (int k = 0; k < O::terms; ++k) {
+    Are[k] = O::load(aop + ((2 * k) * 64 + lane) * 8);
+    Aim[k] = O::load(aop + ((2 * k + 1) * 64 + lane) * 8);
+  }
+  const __amdgpu_buffer_rsrc_t rin = bufio::make_rsrc(a.in, in_bytes);
+  const __amdgpu_buffer_rsrc_t rout = bufio::make_rsrc(a.out, out_bytes);
+  const uint32_t es_in = uint32_t(a.esi) * 4u, es_out = uint32_t(a.eso) * 4u;
+  fill(tile_of(0), 0);
+  if (tile_of(1) < ntiles) fill(tile_of(1), 1);
+  __syncthreads();
+
+  C2<float> in[4][4];
+  // inputs bj(g) + 64 (r0 + i) of line ml; a missing line reads zeros
+  auto load_tile = [&](int s, int64_t tile) {
+    const bool ok = tile * L + ml < a.nlines;
+    const uint32_t vo = ok ? uint32_t(s_oi[s][ml]) * 4u + uint32_t(bj(0) + 64 * r0) * es_in
+                           : in_bytes;
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        in[g][i] = bufio::bload(rin, vo, uint32_t(bj(g) - bj(0) + 64 * i) * es_in);
+  };
+  load_tile(0, tile_of(0));
+
+  for (int64_t k = 0;; ++k) {
+    int z;  // opaque zero: keeps the LDS twiddle reads in the loop
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+    const int s = int(k % 3);
+    const int64_t tile = tile_of(k);
+    int ex = 0;  // FP16: exponent of the line's largest input
+    if constexpr (MODE == MODE_F16) {
+      float m = 0.f;
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) m = fmaxf(m, fmaxf(fabsf(in[g][i].x), fabsf(in[g][i].y)));
+      m = fmaxf(m, __shfl_xor(m, 16));
+      m = fmaxf(m, __shfl_xor(m, 32));
+      if constexpr (PASS_A) {  // line = column: over the 16 waves
+        if (kg == 0) s_max[wave][col] = m;
+        __syncthreads();
+        m = 0.f;
+#pragma unroll
+        for (int w = 0; w < 16; ++w) m = fmaxf(m, s_max[w][col]);
+      } else {
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) m = fmaxf(m, __shfl_xor(m, o));
+      }
+      (void)frexpf(m, &ex);
+      if (m == 0.f) ex = 10;
+    }
+    // ---- stage 0 (radix 16) on the matrix cores; output r of butterfly j
+    // lands at 16 j + r, stage 1's input j / 4, twiddled here by
+    // W_256^((j / 4) r)
+    {
+      const float sc0 = MODE == MODE_F16 ? ldexpf(1.f, 10 - ex) : 1.f;
+      const float inv0 = MODE == MODE_F16 ? ldexpf(1.f, ex - 10) : 1.f;
+      C2<float> *line = lds + ml * P;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        f4 dre, dim;
+        dft16_mma<MODE>(in[g], sc0, Are, Aim, dre, dim);
+        const int j = bj(g);
+        const C2<float> *tw = st1l + (j >> 2) * 16 + z;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = 4 * kg + q;
+          line[padf(j * 16 + r)] = cmul(C2<float>{dre[q] * inv0, dim[q] * inv0}, tw[r]);
+        }
+      }
+    }
+    if (tile_of(k + 2) < ntiles) fill(tile_of(k + 2), int((k + 2) % 3));
+    const bool more = tile_of(k + 1) < ntiles;
+    if (more) load_tile(int((k + 1) % 3), tile_of(k + 1));
+    if constexpr (PASS_A) __syncthreads();
+    else {
+      __builtin_amdgcn_wave_barrier();
+      asm volatile("" ::: "memory");
+    }
+    // ---- stage 1 (radix 16, NS 16), inputs already twiddled.  Output k of
+    // butterfly j goes back to the slot of its input k (j + 64 k): the
+    // positions a wave writes are the ones it read, so no barrier separates
+    // the reads from the writes; stage 2 reads the Stockham order through
+    // that map.
+    {
+      const float sc1 = MODE == MODE_F16 ? ldexpf(1.f, 6 - ex) : 1.f;
+      const float inv1 = MODE == MODE_F16 ? ldexpf(1.f, ex - 6) : 1.f;
+      C2<float> *ln = lds + ml * P;
+#pragma unroll 1
+      for (int g = 0; g < 4; ++g) {
+        const int j = bj(g);
+        C2<float> x[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[i] = ln[padf(j + 64 * (r0 + i))];
+        f4 dre, dim;
+        dft16_mma<MODE>(x, sc1, Are, Aim, dre, dim);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          ln[padf(j + 64 * (4 * kg + q))] = C2<float>{dre[q] * inv1, dim[q] * inv1};
+      }
+    }
+    __syncthreads();
+    // ---- stage 2 (radix 4, NS 256) on the VALU, natural-order bins out
+    if (tile * L + l2 < a.nlines) {
+      const C2<float> *ln = lds + l2 * P;
+      const uint32_t vo = uint32_t(s_oo[s][l2]) * 4u + uint32_t(i2) * es_out;
+      const int aux = s_aux[s][l2];
+#pragma unroll 1
+      for (int q = 0; q < 4; ++q) {
+        const int j = i2 + 64 * q;
+        C2<float> u[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) u[r] = ln[padf(16 * r + (j & 15) + 64 * (j >> 4))];
+        const C2<float> w1 = st2l[j + z], w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+        u[1] = cmul(u[1], w1);
+        u[2] = cmul(u[2], w2);
+        u[3] = cmul(u[3], w3);
+        dft4(u, dir);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int kb = j + 256 * r;
+          C2<float> xo;
+          if constexpr (PASS_A) {
+            const unsigned e = unsigned(kb) * unsigned(aux);
+            const unsigned mask = (1u << a.shift) - 1u;
+            const C2<float> w = cmul(twl[e & mask], twl[a.twlds_lo + int(e >> a.shift)]);
+            xo = cmul(u[r], dirw(w, dir));
+          } else {
+            xo = a.fct * u[r];
+          }
+          bufio::bstore(rout, vo, uint32_t(64 * q + 256 * r) * es_out, xo);
+        }
+      }
+    }
+    if (!more) break;
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------- host side
 
 // x rounded to `bits` significant bits (ties to even), with a quantum of at
@@ -562,6 +762,61 @@ namespace {
 std::mutex g_pass_attr_mu;
 std::set<std::pair<int, const void *>> g_pass_attr;  // (device, kernel)
 
+void set_lds_attr(const void *fn) {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_pass_attr_mu);
+  if (!g_pass_attr.count({dev, fn})) {
+    hipFuncAttributes fa;
+    HIP_CHECK(hipFuncGetAttributes(&fa, fn));
+    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  int(lds_limit() - fa.sharedSizeBytes)));
+    g_pass_attr.insert({dev, fn});
+  }
+}
+
+// The persistent pipelined form: spans below 2^31 bytes with non-negative
+// strides (buffer offsets), pass A's four-step tables in LDS beside the
+// tile and the stage tables; otherwise the caller takes the one-tile form.
+bool launch_pass1024_pipe(int mode, bool pass_a, const dev::LineArgs<float> &a0, hipStream_t s) {
+  const int64_t si = bufio::line_span(a0, 1024, true), so = bufio::line_span(a0, 1024, false);
+  if (si < 0 || so < 0 || si * 4 >= (int64_t(1) << 31) - (int64_t(8) << 20) ||
+      so * 4 >= (int64_t(1) << 31))
+    return false;
+  dev::LineArgs<float> a = a0;
+  const int64_t ntiles = (a.nlines + kPipeL - 1) / kPipeL;
+  if (ntiles <= 0) return true;
+  size_t lds = size_t(kPipeL * kPipeP + kPipeSt1 + kPipeSt2) * sizeof(C2<float>);
+  a.twlds_lo = a.twlds_hi = 0;
+  if (pass_a) {
+    const int64_t big = int64_t(1024) * a.other;
+    const int64_t lo_n = int64_t(1) << a.shift, hi_n = (big + lo_n - 1) / lo_n;
+    const size_t extra = size_t(lo_n + hi_n) * sizeof(C2<float>);
+    if (big >= (int64_t(1) << 31) || lds + extra + 2048 > lds_limit()) return false;
+    a.twlds_lo = int(lo_n);
+    a.twlds_hi = int(hi_n);
+    lds += extra;
+  }
+  const void *fn = nullptr;
+#define RFH_PIPE_FN(M)                                                                     \
+  fn = pass_a ? reinterpret_cast<const void *>(&k_pass1024_mfma_pipe<M, true>)            \
+              : reinterpret_cast<const void *>(&k_pass1024_mfma_pipe<M, false>)
+  switch (mode) {
+    case MODE_F16: RFH_PIPE_FN(MODE_F16); break;
+    case MODE_BF16_3: RFH_PIPE_FN(MODE_BF16_3); break;
+    default: RFH_PIPE_FN(MODE_BF16_2); break;
+  }
+#undef RFH_PIPE_FN
+  set_lds_attr(fn);
+  const float *aop = operands(mode, a.dir < 0 ? -1 : 1);
+  int64_t grid = std::min<int64_t>(ntiles, bufio::device_cus());
+  if (grid >= 8) grid -= grid % 8;
+  uint32_t in_bytes = uint32_t(si * 4), out_bytes = uint32_t(so * 4);
+  void *args[] = {&a, &aop, const_cast<int64_t *>(&ntiles), &in_bytes, &out_bytes};
+  HIP_CHECK(hipLaunchKernel(fn, dim3(unsigned(grid)), dim3(1024), args, lds, s));
+  return true;
+}
+
 // The four-step passes of complex64 lines of 1024 on the matrix cores.
 bool launch_pass1024(int mode, int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out,
                      hipStream_t s) {
@@ -581,6 +836,7 @@ bool launch_pass1024(int mode, int op, const dev::LineArgs<float> &a, bool lf_in
     default: RFH_PASS_FN(MODE_BF16_2); break;
   }
 #undef RFH_PASS_FN
+  if (a.sub == 0 && launch_pass1024_pipe(mode, pass_a, a, s)) return true;
   size_t lds = size_t(16) * (1024 + 64 + 1) * sizeof(C2<float>);
   dev::LineArgs<float> la = a;
   la.twlds_lo = la.twlds_hi = 0;
@@ -594,18 +850,7 @@ bool launch_pass1024(int mode, int op, const dev::LineArgs<float> &a, bool lf_in
       lds += extra;
     }
   }
-  {
-    int dev = 0;
-    HIP_CHECK(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lk(g_pass_attr_mu);
-    if (!g_pass_attr.count({dev, fn})) {
-      hipFuncAttributes fa;
-      HIP_CHECK(hipFuncGetAttributes(&fa, fn));
-      HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    int(lds_limit() - fa.sharedSizeBytes)));
-      g_pass_attr.insert({dev, fn});
-    }
-  }
+  set_lds_attr(fn);
   const float *aop = operands(mode, a.dir < 0 ? -1 : 1);
   const unsigned grid = unsigned((a.nlines + 15) / 16);
   void *args[] = {&la, &aop};

@@ -22,6 +22,7 @@
 #include <mutex>
 #include <set>
 
+#include "bufio.hpp"
 #include "passes.hpp"
 
 namespace rfh {
@@ -30,29 +31,11 @@ using namespace dev;
 
 constexpr int kN = 1024, kL = 16, kP = kN + kN / 16 + 1;  // padded line pitch
 constexpr int kSt1 = 16 * 16, kSt2 = 256;  // stage twiddles kept in LDS
-typedef __attribute__((ext_vector_type(2))) unsigned u2;
+using bufio::bload;
+using bufio::bstore;
+using bufio::make_rsrc;
 
 __device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, int(bytes),
-                                           0x00020000);
-}
-__device__ __forceinline__ C2<float> bload(__amdgpu_buffer_rsrc_t r, uint32_t voff,
-                                           uint32_t soff) {
-  const u2 v = __builtin_amdgcn_raw_buffer_load_b64(r, int(voff), int(soff), 0);
-  // components through scalar temporaries: a bit cast of a vector element
-  // expression itself would read the vector's first element
-  const unsigned lo = v.x, hi = v.y;
-  return C2<float>{__builtin_bit_cast(float, lo), __builtin_bit_cast(float, hi)};
-}
-__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
-                                       C2<float> x) {
-  u2 v;
-  v[0] = __builtin_bit_cast(unsigned, x.x);
-  v[1] = __builtin_bit_cast(unsigned, x.y);
-  __builtin_amdgcn_raw_buffer_store_b64(v, r, int(voff), int(soff), 0);
-}
 
 // The exchanges of stages 0 and 1 stay inside a line.  In pass B a wave owns
 // its line there, and a wave's LDS operations complete in issue order, so
@@ -177,17 +160,18 @@ __global__ void __launch_bounds__(1024, 1)
     const bool more = tile_of(k + 1) < ntiles;
     if (more) load_tile(int((k + 1) % 3), tile_of(k + 1));
     line_sync<PASS_A>();
-    // ---- stage 1 (radix 16, NS 16)
+    // ---- stage 1 (radix 16, NS 16).  Output k of butterfly j goes back to
+    // the slot of its input k (j + 64 k), which only this thread reads, so
+    // no barrier separates the reads from the writes; stage 2 reads the
+    // Stockham order (j / 16) 256 + j % 16 + 16 k through that map.
     {
       C2<float> w[16];
       C2<float> *ln = lds + l0 * kP;
-      const int j = i0, kk = j & 15;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) w[r] = ln[pad(j + 64 * r)];  // twiddled in stage 0
+      for (int r = 0; r < 16; ++r) w[r] = ln[pad(i0 + 64 * r)];  // twiddled in stage 0
       dft<16>(w, dir, static_cast<const C2<float> *>(nullptr));
-      line_sync<PASS_A>();  // every read of stage 1 is done
 #pragma unroll
-      for (int r = 0; r < 16; ++r) ln[pad((j - kk) * 16 + kk + r * 16)] = w[r];
+      for (int r = 0; r < 16; ++r) ln[pad(i0 + 64 * r)] = w[r];
     }
     __syncthreads();
     // ---- stage 2 (radix 4, NS 256), natural-order bins to global memory
@@ -200,7 +184,7 @@ __global__ void __launch_bounds__(1024, 1)
         const int j = i2 + 64 * q;
         C2<float> u[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) u[r] = ln[pad(j + 256 * r)];
+        for (int r = 0; r < 4; ++r) u[r] = ln[pad(16 * r + (j & 15) + 64 * (j >> 4))];
         const C2<float> w1 = st2l[j + z], w2 = cmul(w1, w1), w3 = cmul(w2, w1);
         u[1] = cmul(u[1], w1);
         u[2] = cmul(u[2], w2);
@@ -234,33 +218,6 @@ __global__ void __launch_bounds__(1024, 1)
 std::mutex g_mu;
 std::set<std::pair<int, const void *>> g_attr;  // (device, kernel) with the LDS limit raised
 
-// largest scalar offset reached by the line set on one side, or -1 when a
-// stride is negative
-int64_t span_of(const LineArgs<float> &a, bool in) {
-  int64_t s = 0;
-  const int64_t es = in ? a.esi : a.eso;
-  if (es < 0) return -1;
-  s += int64_t(kN - 1) * es + 2;
-  for (int d = 0; d < a.nb; ++d) {
-    const int64_t st = in ? a.bsi[d] : a.bso[d];
-    if (st < 0 || a.bdim[d] < 1) return -1;
-    s += (a.bdim[d] - 1) * st;
-  }
-  return s;
-}
-
-int cus() {
-  static const int n = [] {
-    int dev = 0, v = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        v <= 0)
-      return 256;
-    return v;
-  }();
-  return n;
-}
-
 }  // namespace
 
 bool launch_pass1024(int op, const LineArgs<float> &a0, bool lf_in, bool lf_out, bool vin,
@@ -276,7 +233,7 @@ bool launch_pass1024(int op, const LineArgs<float> &a0, bool lf_in, bool lf_out,
     return false;
   const bool rot = (pass_a ? a0.rot_in : a0.rot_out) != 0;
   if (pass_a ? !(lf_in && lf_out) : !(op == OP_PASSB && !lf_in && lf_out)) return false;
-  const int64_t si = span_of(a0, true), so = span_of(a0, false);
+  const int64_t si = bufio::line_span(a0, kN, true), so = bufio::line_span(a0, kN, false);
   if (si < 0 || so < 0 || si * 4 >= (int64_t(1) << 31) - (int64_t(8) << 20) ||
       so * 4 >= (int64_t(1) << 31))
     return false;
@@ -311,7 +268,7 @@ bool launch_pass1024(int op, const LineArgs<float> &a0, bool lf_in, bool lf_out,
       g_attr.insert({dev, fn});
     }
   }
-  int64_t grid = std::min<int64_t>(ntiles, cus());
+  int64_t grid = std::min<int64_t>(ntiles, bufio::device_cus());
   if (grid >= 8) grid -= grid % 8;
   const uint32_t in_bytes = uint32_t(si * 4), out_bytes = uint32_t(so * 4);
   const dim3 gd = dim3(unsigned(grid)), bd = dim3(1024);
