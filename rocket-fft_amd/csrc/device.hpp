@@ -1028,6 +1028,11 @@ extern bool g_fast_enabled;
 bool launch_pass1024(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out, bool vin,
                      bool vout, hipStream_t s);
 
+// Persistent software-pipelined FP32 transform of contiguous complex64 lines
+// of 4096 (pipe4096.hip); false when the pass does not match.
+bool launch_pipe4096(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out, bool vin,
+                     bool vout, hipStream_t s);
+
 // Split-precision MFMA kernels (mfma.hip); false when the mode is off or the
 // pass is not a contiguous complex64 line of 4096.
 bool launch_mfma(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out, bool vin,

@@ -331,6 +331,10 @@ void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
       t_last_family = "k_pass1024";
       return;
     }
+    if (launch_pipe4096(ps.op, a, a.lfast_in, a.lfast_out, vin, vout, s)) {
+      t_last_family = "k_pipe4096";
+      return;
+    }
   }
   if (launch_fast<T>(ps.op, a, a.lfast_in, a.lfast_out, vin, vout, s)) {
     t_last_family = "k_fast";

@@ -14,7 +14,7 @@ PKG_ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(PKG_ROOT, "csrc")
 BUILD = os.path.join(PKG_ROOT, "_build")
 LIB = os.path.join(HERE, "librocketfft_hip.so")
-SOURCES = ["plan.hip", "kernels.hip", "fast.hip", "mfma.hip", "pass1024.hip", "ragged.hip",
+SOURCES = ["plan.hip", "kernels.hip", "fast.hip", "mfma.hip", "pass1024.hip", "pipe4096.hip", "ragged.hip",
            "engine.hip", "lineops.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -112,6 +112,29 @@ def test_two_pass_2p20_pipelined(lines):
     check("c2c", x, x, np.array([1]), False, 1.0 / n)
 
 
+@pytest.mark.parametrize("lines", [2048, 2051])
+def test_c2c_4096_pipelined(lines):
+    """complex64 lines of 4096, enough of them for the persistent pipelined
+    kernel (4-line tiles, at least two per CU; checked through the per-launch
+    timing labels); 2051 lines leave a partial last tile.  Both directions
+    against the oracle."""
+    import torch
+    n = 4096
+    rs = np.random.RandomState(lines)
+    x = crandn(rs, (lines, n), np.complex64)
+    R.kernel_timing_report()
+    R.kernel_timing(True)
+    try:
+        check("c2c", x, x, np.array([1]), True, 1.0)
+    finally:
+        torch.cuda.synchronize()
+        R.kernel_timing(False)
+    labels = R.kernel_timing_report()
+    print("labels", sorted(labels), flush=True)
+    assert any(k.startswith("k_pipe4096 c2c n=4096 c64") for k in labels)
+    check("c2c", x, x, np.array([1]), False, 1.0 / n)
+
+
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 @pytest.mark.parametrize("n", [8193, 32749, 65537])
 def test_bluestein_pow2_rows(dtype, n):

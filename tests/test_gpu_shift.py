@@ -66,6 +66,16 @@ def test_fftshift_one_axis(dtype, n):
     check_case(x, [1], True, 1.0, None, [n // 2])              # fftshift(fft(x))
 
 
+def test_fftshift_4096_pipelined_complex64():
+    """2051 lines of 4096: the persistent pipelined kernel with rotated loads
+    and stores (partial last tile), bit for bit against roll(unfused)."""
+    n = 4096
+    rs = np.random.RandomState(5)
+    x = crandn(rs, (2051, n), np.complex64)
+    check_case(x, [1], True, 1.0, [-(n // 2)], [n // 2])
+    check_case(x, [1], False, 1.0 / n, [n // 2 + 5], [-(n // 2) + 3])
+
+
 def test_fftshift_2p20_complex64():
     n = 1 << 20
     rs = np.random.RandomState(20)
