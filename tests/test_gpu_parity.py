@@ -92,8 +92,8 @@ def test_short_lines_partial_tiles(dtype, n, lines):
 def test_two_pass_2p20_pipelined(lines):
     """complex64 2^20 = 1024 x 1024: both four-step passes run on the
     persistent pipelined kernel (checked through the per-launch timing
-    labels); 17 lines make a chunk of 16 and a chunk of 1 (64 tiles, fewer
-    than the CUs).  Both directions against the oracle."""
+    labels); 3 lines are 192 tiles, fewer than the CUs.  Both directions
+    against the oracle."""
     import torch
     n = 1 << 20
     rs = np.random.RandomState(lines)
@@ -342,11 +342,12 @@ def test_separable_hartley_strided_axis0():
     check("r2r_fftpack", x, x, np.array([0]), True, True, 1.0)
 
 
-@pytest.mark.parametrize("rows,n,dtype", [(40, 1 << 18, np.complex64), (24, 1 << 18, np.complex128),
-                                          (80, 65537, np.complex64)])
+@pytest.mark.parametrize("rows,n,dtype", [(80, 1 << 18, np.complex64), (40, 1 << 18, np.complex128),
+                                          (160, 65537, np.complex64), (65, 1 << 20, np.complex64)])
 def test_two_pass_chunked(rows, n, dtype):
-    """Batches whose two-pass intermediate exceeds one 64 MiB chunk run chunk
-    by chunk; checked against an FP64 transform (torch.fft, complex128)."""
+    """Batches whose two-pass intermediate exceeds one chunk (128 MiB; 512 MiB
+    for the complex64 1024 x 1024 plans) run chunk by chunk, the last chunk
+    short; checked against an FP64 transform (torch.fft, complex128)."""
     import torch
     g = torch.Generator(device="cuda")
     g.manual_seed(rows)
