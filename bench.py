@@ -81,13 +81,15 @@ def err_vs_fp64(torch, x, y, rows=None):
     return float(np.sqrt(num / den))
 
 
-def graph_time(torch, fn, stream, reps, batches=5):
-    """Capture fn once and replay it: `batches` timed batches of `reps`
-    replays (events on `stream`); median over batches of the mean ms per
-    replay."""
+def graph_time(torch, fn, stream, reps, batches=5, inner=1):
+    """Capture fn `inner` times in one graph and replay it: `batches` timed
+    batches of `reps` replays (events on `stream`); median over batches of
+    the mean ms per call of fn (inner > 1 keeps short calls from being
+    timed at the host's graph-launch rate)."""
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g, stream=stream):
-        fn()
+        for _ in range(inner):
+            fn()
     torch.cuda.synchronize()
     g.replay()
     torch.cuda.synchronize()
@@ -100,7 +102,7 @@ def graph_time(torch, fn, stream, reps, batches=5):
                 g.replay()
             e1.record(stream)
         torch.cuda.synchronize()
-        times.append(e0.elapsed_time(e1) / reps)
+        times.append(e0.elapsed_time(e1) / (reps * inner))
     return float(np.median(times)), g
 
 
@@ -360,8 +362,9 @@ def real3d_config(torch, R, stream, reps):
 
 def lengths_config(torch, R, stream, reps):
     """Mixed-radix and prime lengths, batch 256, complex64 and complex128:
-    forward c2c timed (graph replays), forward error against numpy's FP64
-    transform on the host, and the forward -> inverse (fct = 1/N) round trip."""
+    forward c2c timed (ten calls per captured graph, replayed), forward error
+    against numpy's FP64 transform on the host, and the forward -> inverse
+    (fct = 1/N) round trip."""
     out = {}
     rs = np.random.RandomState(3)
     ax = np.array([1])
@@ -381,7 +384,8 @@ def lengths_config(torch, R, stream, reps):
             ref = np.fft.fft(xh.astype(np.complex128), axis=1) if npdt == np.complex64 else want
             fwd_err = float(np.linalg.norm(b.cpu().numpy() - ref) / np.linalg.norm(ref))
             rt_err = float(np.linalg.norm(c.cpu().numpy() - xh) / np.linalg.norm(xh))
-            ms, _ = graph_time(torch, lambda: R.c2c(a, b, ax, True, 1.0, 1), stream, reps)
+            ms, _ = graph_time(torch, lambda: R.c2c(a, b, ax, True, 1.0, 1), stream,
+                               max(1, reps // 10), inner=10)
             nbytes = 2 * a.numel() * a.element_size()
             out[f"{name}_n{n}"] = {"ms": round(ms, 4), "transforms_per_s": round(256 / ms * 1e3),
                                    "GB/s": round(nbytes / ms / 1e6, 1),
