@@ -1033,10 +1033,11 @@ bool launch_pass1024(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_
 bool launch_pipe4096(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out, bool vin,
                      bool vout, hipStream_t s);
 
-// Split-precision MFMA kernels (mfma.hip); false when the mode is off or the
-// pass is not a contiguous complex64 line of 4096.
-bool launch_mfma(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out, bool vin,
-                 bool vout, hipStream_t s);
+// Split-precision MFMA kernels (mfma.hip): the kernel family launched, or
+// nullptr when the mode is off or the pass has no matrix-core form
+// (contiguous complex64 lines of 4096, the 1024-point four-step passes).
+const char *launch_mfma(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out,
+                        bool vin, bool vout, hipStream_t s);
 void mfma_probe(int bf, const float *a, const float *b, float *d, hipStream_t s);
 extern int g_precision_mode;
 // launches of the matrix-core kernels in precision mode `mode` (0: all modes)

@@ -323,8 +323,8 @@ void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
   else
     vout = pairs_aligned(g.out, sizeof(T), g.eso, g.bso, g.nb);
   if constexpr (sizeof(T) == 4) {
-    if (launch_mfma(ps.op, a, a.lfast_in, a.lfast_out, vin, vout, s)) {
-      t_last_family = "mfma";
+    if (const char *fam = launch_mfma(ps.op, a, a.lfast_in, a.lfast_out, vin, vout, s)) {
+      t_last_family = fam;
       return;
     }
     if (launch_pass1024(ps.op, a, a.lfast_in, a.lfast_out, vin, vout, s)) {

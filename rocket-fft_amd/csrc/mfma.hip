@@ -672,6 +672,177 @@ __global__ void __launch_bounds__(1024, 1)
   }
 }
 
+// Contiguous complex64 lines of 4096 on the matrix cores, persistent and
+// software-pipelined like the FP32 form (pipe4096.hip): one 1024-thread
+// workgroup per CU walks tiles of four lines (four waves per line); a wave
+// runs 64 butterflies of its line per stage, in four groups of sixteen (the
+// matrix column being the butterfly).  The next tile's loads go out after
+// stage 0.  Each stage's twiddles are applied as the stage before writes:
+// stage 1's W_256^(r k) from an LDS table, stage 2's W_4096^(j2 (j / 16)) as
+// A_j W_256^(q (j / 16)) (A_j = W_4096^((j mod 16)(j / 16)) from a second
+// LDS table).  Stage 1 runs in place (each wave writes back exactly the
+// slots it read), and stage 2 stores its bins straight to memory.
+constexpr int kP4L = 4, kP4P = 4096 + 256 + 1, kP4St1 = 256, kP4Lo = 256;
+
+template <int MODE>
+__global__ void __launch_bounds__(1024, 1)
+    k_pipe4096_mfma(const LineArgs<float> a, const float *__restrict__ aop, int64_t ntiles,
+                    uint32_t in_bytes, uint32_t out_bytes) {
+  using O = Ops<MODE>;
+  typedef typename O::V V;
+  constexpr int L = kP4L, P = kP4P, NR = 256;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C2<float> *lds = reinterpret_cast<C2<float> *>(smem_raw);
+  __shared__ int64_t s_oi[3][L], s_oo[3][L];
+  __shared__ float s_max[16];
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63, col = lane & 15, kg = lane >> 4;
+  const int r0 = (kg & 1) * 8 + (kg >= 2 ? 4 : 0);
+  const int l = wave >> 2, wl = wave & 3;  // line of the tile, wave within the line
+  auto bj = [&](int g) { return wl * 64 + g * 16 + col; };
+  const int64_t G = gridDim.x, b = blockIdx.x;
+  const int64_t slot0 = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+  if (slot0 >= ntiles) return;
+  auto tile_of = [&](int64_t k) { return k * G + slot0; };
+  auto fill = [&](int64_t tile, int s) {
+    if (t < L) {
+      const LineOffsets o =
+          group_slot(tile * L, t, L, a.nlines, a.nb, a.bdim, a.bsi, a.bso, a.auxdim);
+      s_oi[s][t] = o.oi;
+      s_oo[s][t] = o.oo;
+    }
+  };
+  const float dir = a.dir;
+  // behind the lines: st1[r 16 + k] = W_256^(r k) (row 0 ones), lo[e] =
+  // W_4096^e (e < 256, the first row of stage 2's table)
+  C2<float> *st1l = lds + L * P;
+  C2<float> *lo = st1l + kP4St1;
+  {
+    const C2<float> *tw1 = a.tw + a.f.st[1].twoff;
+    const C2<float> *tw2 = a.tw + a.f.st[2].twoff;
+    for (int q = t; q < kP4St1 + kP4Lo; q += 1024)
+      st1l[q] = q < 16 ? C2<float>{1.f, 0.f}
+                       : dirw(q < kP4St1 ? tw1[q - 16] : tw2[q - kP4St1], dir);
+  }
+  V Are[O::terms], Aim[O::terms];
+#pragma unroll
+  for (int k = 0; k < O::terms; ++k) {
+    Are[k] = O::load(aop + ((2 * k) * 64 + lane) * 8);
+    Aim[k] = O::load(aop + ((2 * k + 1) * 64 + lane) * 8);
+  }
+  const __amdgpu_buffer_rsrc_t rin = bufio::make_rsrc(a.in, in_bytes);
+  const __amdgpu_buffer_rsrc_t rout = bufio::make_rsrc(a.out, out_bytes);
+  const uint32_t es_in = uint32_t(a.esi) * 4u, es_out = uint32_t(a.eso) * 4u;
+  fill(tile_of(0), 0);
+  if (tile_of(1) < ntiles) fill(tile_of(1), 1);
+  __syncthreads();
+
+  C2<float> in[4][4];
+  // inputs bj(g) + 256 (r0 + i) of line l; a missing line reads zeros
+  auto load_tile = [&](int s, int64_t tile) {
+    const bool ok = tile * L + l < a.nlines;
+    const uint32_t vo = ok ? uint32_t(s_oi[s][l]) * 4u + uint32_t(bj(0) + NR * r0) * es_in
+                           : in_bytes;
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        in[g][i] = bufio::bload(rin, vo, uint32_t(16 * g + NR * i) * es_in);
+  };
+  load_tile(0, tile_of(0));
+
+  for (int64_t k = 0;; ++k) {
+    int z;  // opaque zero: keeps the LDS twiddle reads in the loop
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+    const int s = int(k % 3);
+    const int64_t tile = tile_of(k);
+    C2<float> *ln = lds + l * P;
+    int ex = 0;  // FP16: exponent of the line's largest input
+    if constexpr (MODE == MODE_F16) {
+      float m = 0.f;
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) m = fmaxf(m, fmaxf(fabsf(in[g][i].x), fabsf(in[g][i].y)));
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) m = fmaxf(m, __shfl_xor(m, o));
+      if (lane == 0) s_max[wave] = m;
+      __syncthreads();
+      m = fmaxf(fmaxf(s_max[4 * l], s_max[4 * l + 1]), fmaxf(s_max[4 * l + 2], s_max[4 * l + 3]));
+      (void)frexpf(m, &ex);
+      if (m == 0.f) ex = 10;
+    }
+    // ---- stage 0: output r of butterfly j lands at 16 j + r, stage 1's
+    // input j / 16 of its butterfly 16 (j mod 16) + r, twiddled here by
+    // W_256^((j / 16) r)
+    {
+      const float sc0 = MODE == MODE_F16 ? ldexpf(1.f, 10 - ex) : 1.f;
+      const float inv0 = MODE == MODE_F16 ? ldexpf(1.f, ex - 10) : 1.f;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        f4 dre, dim;
+        dft16_mma<MODE>(in[g], sc0, Are, Aim, dre, dim);
+        const int j = bj(g);
+        const C2<float> *tw = st1l + (j >> 4) * 16 + z;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = 4 * kg + q;
+          ln[padf(j * 16 + r)] = cmul(C2<float>{dre[q] * inv0, dim[q] * inv0}, tw[r]);
+        }
+      }
+    }
+    if (tile_of(k + 2) < ntiles) fill(tile_of(k + 2), int((k + 2) % 3));
+    const bool more = tile_of(k + 1) < ntiles;
+    if (more) load_tile(int((k + 1) % 3), tile_of(k + 1));
+    __syncthreads();
+    // ---- stage 1 (NS 16), inputs already twiddled; output q of butterfly
+    // j goes back to j + 256 q with stage 2's twiddle A_j W_256^(q (j / 16))
+    {
+      const float sc1 = MODE == MODE_F16 ? ldexpf(1.f, 6 - ex) : 1.f;
+      const float inv1 = MODE == MODE_F16 ? ldexpf(1.f, ex - 6) : 1.f;
+#pragma unroll 1
+      for (int g = 0; g < 4; ++g) {
+        const int j = bj(g);
+        C2<float> x[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[i] = ln[padf(j + NR * (r0 + i))];
+        f4 dre, dim;
+        dft16_mma<MODE>(x, sc1, Are, Aim, dre, dim);
+        const C2<float> aj = lo[(j & 15) * (j >> 4) + z];
+        const C2<float> *tw = st1l + (j >> 4) * 16;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int qq = 4 * kg + q;
+          ln[padf(j + NR * qq)] =
+              cmul(C2<float>{dre[q] * inv1, dim[q] * inv1}, cmul(aj, tw[qq]));
+        }
+      }
+    }
+    __syncthreads();
+    // ---- stage 2 (NS 256): butterfly j2 takes Stockham inputs j2 + 256 r
+    // (at 16 r + j2 mod 16 + 256 (j2 / 16)) and stores bins j2 + 256 r
+    if (tile * L + l < a.nlines) {
+      const float sc2 = MODE == MODE_F16 ? ldexpf(1.f, 2 - ex) : 1.f;
+      const float inv2 = MODE == MODE_F16 ? ldexpf(1.f, ex - 2) : 1.f;
+      const uint32_t vo = uint32_t(s_oo[s][l]) * 4u + uint32_t(bj(0) + NR * 4 * kg) * es_out;
+#pragma unroll 1
+      for (int g = 0; g < 4; ++g) {
+        const int j2 = bj(g);
+        C2<float> x[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[i] = ln[padf(16 * (r0 + i) + (j2 & 15) + NR * (j2 >> 4))];
+        f4 dre, dim;
+        dft16_mma<MODE>(x, sc2, Are, Aim, dre, dim);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          bufio::bstore(rout, vo, uint32_t(16 * g + NR * q) * es_out,
+                        C2<float>{a.fct * dre[q] * inv2, a.fct * dim[q] * inv2});
+      }
+    }
+    if (!more) break;
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------- host side
 
 // x rounded to `bits` significant bits (ties to even), with a quantum of at
@@ -817,15 +988,16 @@ bool launch_pass1024_pipe(int mode, bool pass_a, const dev::LineArgs<float> &a0,
   return true;
 }
 
-// The four-step passes of complex64 lines of 1024 on the matrix cores.
-bool launch_pass1024(int mode, int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out,
-                     hipStream_t s) {
+// The four-step passes of complex64 lines of 1024 on the matrix cores; the
+// kernel family launched, or nullptr.
+const char *launch_pass1024(int mode, int op, const dev::LineArgs<float> &a, bool lf_in,
+                            bool lf_out, hipStream_t s) {
   const bool pass_a = op == OP_PASSA && lf_in && lf_out && !(a.sub & (SUB_CHIRP | SUB_FINAL));
   const bool pass_b = op == OP_PASSB && !lf_in && lf_out &&
                       !(a.sub & (SUB_KERNEL | SUB_FINAL));
-  if (!pass_a && !pass_b) return false;
+  if (!pass_a && !pass_b) return nullptr;
   if (a.f.nst != 3 || a.f.st[0].radix != 16 || a.f.st[1].radix != 16 || a.f.st[2].radix != 4)
-    return false;
+    return nullptr;
   const void *fn = nullptr;
 #define RFH_PASS_FN(M)                                                                     \
   fn = pass_a ? reinterpret_cast<const void *>(&k_pass1024_mfma<M, true>)                 \
@@ -836,7 +1008,7 @@ bool launch_pass1024(int mode, int op, const dev::LineArgs<float> &a, bool lf_in
     default: RFH_PASS_FN(MODE_BF16_2); break;
   }
 #undef RFH_PASS_FN
-  if (a.sub == 0 && launch_pass1024_pipe(mode, pass_a, a, s)) return true;
+  if (a.sub == 0 && launch_pass1024_pipe(mode, pass_a, a, s)) return "k_pass1024_mfma_pipe";
   size_t lds = size_t(16) * (1024 + 64 + 1) * sizeof(C2<float>);
   dev::LineArgs<float> la = a;
   la.twlds_lo = la.twlds_hi = 0;
@@ -855,24 +1027,58 @@ bool launch_pass1024(int mode, int op, const dev::LineArgs<float> &a, bool lf_in
   const unsigned grid = unsigned((a.nlines + 15) / 16);
   void *args[] = {&la, &aop};
   HIP_CHECK(hipLaunchKernel(fn, dim3(grid), dim3(1024), args, lds, s));
+  return "k_pass1024_mfma";
+}
+
+// Contiguous complex64 lines of 4096 on the persistent pipelined kernel:
+// sets of at least two tiles per CU, spans below 2^31 bytes with
+// non-negative strides; otherwise the caller takes the one-line form.
+bool launch_pipe4096_mfma(int mode, const dev::LineArgs<float> &a, hipStream_t s) {
+  const int64_t ntiles = (a.nlines + kP4L - 1) / kP4L;
+  const int cus = bufio::device_cus();
+  if (ntiles < 2 * int64_t(cus)) return false;
+  const int64_t si = bufio::line_span(a, 4096, true), so = bufio::line_span(a, 4096, false);
+  if (si < 0 || so < 0 || si * 4 >= (int64_t(1) << 31) - (int64_t(8) << 20) ||
+      so * 4 >= (int64_t(1) << 31))
+    return false;
+  const void *fn = mode == MODE_F16 ? reinterpret_cast<const void *>(&k_pipe4096_mfma<MODE_F16>)
+                   : mode == MODE_BF16_3
+                       ? reinterpret_cast<const void *>(&k_pipe4096_mfma<MODE_BF16_3>)
+                       : reinterpret_cast<const void *>(&k_pipe4096_mfma<MODE_BF16_2>);
+  const size_t lds = size_t(kP4L * kP4P + kP4St1 + kP4Lo) * sizeof(C2<float>);
+  if (lds + 2048 > lds_limit()) return false;
+  set_lds_attr(fn);
+  const float *aop = operands(mode, a.dir < 0 ? -1 : 1);
+  int64_t grid = std::min<int64_t>(ntiles, cus);
+  if (grid >= 8) grid -= grid % 8;
+  uint32_t in_bytes = uint32_t(si * 4), out_bytes = uint32_t(so * 4);
+  dev::LineArgs<float> la = a;
+  int64_t nt = ntiles;
+  void *args[] = {&la, &aop, &nt, &in_bytes, &out_bytes};
+  HIP_CHECK(hipLaunchKernel(fn, dim3(unsigned(grid)), dim3(1024), args, lds, s));
   return true;
 }
 }  // namespace
 
 // Complex64 lines of 4096, contiguous, one line per group, and the four-step
 // passes of complex64 lines of 1024: the MFMA form.
-bool launch_mfma(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out, bool vin,
-                 bool vout, hipStream_t s) {
-  if (!g_precision_mode || !vin || !vout) return false;
-  if (a.rot_in || a.rot_out) return false;  // rotated lines run on the FP32 kernels
+const char *launch_mfma(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out,
+                        bool vin, bool vout, hipStream_t s) {
+  if (!g_precision_mode || !vin || !vout) return nullptr;
+  if (a.rot_in || a.rot_out) return nullptr;  // rotated lines run on the FP32 kernels
   if (a.f.n == 1024 && (op == OP_PASSA || op == OP_PASSB)) {
-    if (!launch_pass1024(g_precision_mode, op, a, lf_in, lf_out, s)) return false;
-    g_mfma_launches[g_precision_mode].fetch_add(1);
-    return true;
+    const char *fam = launch_pass1024(g_precision_mode, op, a, lf_in, lf_out, s);
+    if (fam) g_mfma_launches[g_precision_mode].fetch_add(1);
+    return fam;
   }
-  if (op != OP_C2C || a.f.n != 4096 || lf_in || lf_out) return false;
-  if (a.f.nst != 3 || a.esi != 2 || a.eso != 2 || a.nb > 1) return false;
+  if (op != OP_C2C || a.f.n != 4096 || lf_in || lf_out) return nullptr;
+  if (a.f.nst != 3 || a.sub != 0) return nullptr;
   const int mode = g_precision_mode;
+  if (launch_pipe4096_mfma(mode, a, s)) {
+    g_mfma_launches[mode].fetch_add(1);
+    return "k_pipe4096_mfma";
+  }
+  if (a.esi != 2 || a.eso != 2 || a.nb > 1) return nullptr;
   const float *aop = operands(mode, a.dir < 0 ? -1 : 1);
   const unsigned grid = unsigned(a.nlines);
   switch (mode) {
@@ -883,7 +1089,7 @@ bool launch_mfma(int op, const dev::LineArgs<float> &a, bool lf_in, bool lf_out,
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) RFH_THROW(2, std::string("mfma launch: ") + hipGetErrorString(e));
   g_mfma_launches[mode].fetch_add(1);
-  return true;
+  return "k_fft4096_mfma";
 }
 
 void mfma_probe(int bf, const float *a, const float *b, float *d, hipStream_t s) {

@@ -80,6 +80,40 @@ def test_mfma_4096_vs_fp64(mode, tol, kind):
             assert errs[(mode, fwd)] > 5 * errs[("fp32", fwd)]
 
 
+@pytest.mark.parametrize("kind", ["normal", "loguniform"])
+def test_mfma_4096_pipelined_vs_fp64(kind):
+    """2051 lines of 4096 (the persistent pipelined matrix-core kernel: four
+    lines per tile, at least two tiles per CU, a partial last tile) in every
+    mode, both directions, against the long-double oracle; the per-launch
+    timing labels name the kernel family."""
+    rs = np.random.RandomState(6)
+    x = inputs(kind, 2051, rs)
+    want = {fwd: oracle_out("c2c", x.astype(np.complex128), x.astype(np.complex128),
+                            np.array([1]), fwd, fct)
+            for fwd, fct in ((True, 1.0), (False, 1.0 / 4096))}
+    for mode, tol in (("mfma_fp16", 1e-6), ("mfma_bf16", 1e-6), ("mfma_bf16_2split", 1e-4)):
+        prev = R.set_precision(mode)
+        try:
+            for fwd, fct in ((True, 1.0), (False, 1.0 / 4096)):
+                t = torch.from_numpy(x).cuda()
+                y = torch.empty_like(t)
+                R.kernel_timing_report()
+                R.kernel_timing(True)
+                try:
+                    R.c2c(t, y, np.array([1]), fwd, fct, 1)
+                    torch.cuda.synchronize()
+                finally:
+                    R.kernel_timing(False)
+                labels = R.kernel_timing_report()
+                assert any(k.startswith("k_pipe4096_mfma c2c n=4096") for k in labels), labels
+                e = rel_l2(y.cpu().numpy(), want[fwd])
+                print(f"mode={mode} pipelined input={kind} forward={fwd}: rel-L2 vs FP64 = "
+                      f"{e:.3e}", flush=True)
+                assert e <= tol
+        finally:
+            R.set_precision(prev)
+
+
 @pytest.mark.parametrize("mode", ["mfma_fp16", "mfma_bf16", "mfma_bf16_2split"])
 @pytest.mark.parametrize("kind", ["normal", "loguniform"])
 def test_mfma_2p20_vs_fp64(mode, kind):
