@@ -519,23 +519,6 @@ __global__ void __launch_bounds__(1024, 1)
   C2<float> *st1l = lds + L * P;
   C2<float> *st2l = st1l + kPipeSt1;
   C2<float> *twl = st2l + kPipeSt2;
-  {
-    const C2<float> *tw1 = a.tw + a.f.st[1].twoff;
-    const C2<float> *tw2 = a.tw + a.f.st[2].twoff;
-    for (int q = t; q < kPipeSt1 + kPipeSt2; q += 1024)
-      st1l[q] = q < 16 ? C2<float>{1.f, 0.f}
-                       : dirw(q < kPipeSt1 ? tw1[q - 16] : tw2[q - kPipeSt1], dir);
-  }
-  if (PASS_A) {
-    const int nt = a.twlds_lo + a.twlds_hi;
-    for (int q = t; q < nt; q += 1024) twl[q] = q < a.twlds_lo ? a.t0[q] : a.t1[q - a.twlds_lo];
-  }
-  V Are[O::terms], Aim[O::terms];
-#pragma unroll
-  for (int k = 0; k < O::terms; ++k) {
-    Are[k] = O::load(aop + ((2 * k) * 64 + lane) * 8);
-    Aim[k] = O::load(aop + ((2 * k + 1) * 64 + lane) * 8);
-  }
   const __amdgpu_buffer_rsrc_t rin = bufio::make_rsrc(a.in, in_bytes);
   const __amdgpu_buffer_rsrc_t rout = bufio::make_rsrc(a.out, out_bytes);
   const uint32_t es_in = uint32_t(a.esi) * 4u, es_out = uint32_t(a.eso) * 4u;
@@ -556,6 +539,26 @@ __global__ void __launch_bounds__(1024, 1)
         in[g][i] = bufio::bload(rin, vo, uint32_t(bj(g) - bj(0) + 64 * i) * es_in);
   };
   load_tile(0, tile_of(0));
+  // the tables' and operands' loads queue behind the first tile's, so both
+  // are in flight together (the first stage waits for all of them anyway)
+  {
+    const C2<float> *tw1 = a.tw + a.f.st[1].twoff;
+    const C2<float> *tw2 = a.tw + a.f.st[2].twoff;
+    for (int q = t; q < kPipeSt1 + kPipeSt2; q += 1024)
+      st1l[q] = q < 16 ? C2<float>{1.f, 0.f}
+                       : dirw(q < kPipeSt1 ? tw1[q - 16] : tw2[q - kPipeSt1], dir);
+  }
+  if (PASS_A) {
+    const int nt = a.twlds_lo + a.twlds_hi;
+    for (int q = t; q < nt; q += 1024) twl[q] = q < a.twlds_lo ? a.t0[q] : a.t1[q - a.twlds_lo];
+  }
+  V Are[O::terms], Aim[O::terms];
+#pragma unroll
+  for (int k = 0; k < O::terms; ++k) {
+    Are[k] = O::load(aop + ((2 * k) * 64 + lane) * 8);
+    Aim[k] = O::load(aop + ((2 * k + 1) * 64 + lane) * 8);
+  }
+  __syncthreads();
 
   for (int64_t k = 0;; ++k) {
     int z;  // opaque zero: keeps the LDS twiddle reads in the loop
@@ -716,19 +719,6 @@ __global__ void __launch_bounds__(1024, 1)
   // W_4096^e (e < 256, the first row of stage 2's table)
   C2<float> *st1l = lds + L * P;
   C2<float> *lo = st1l + kP4St1;
-  {
-    const C2<float> *tw1 = a.tw + a.f.st[1].twoff;
-    const C2<float> *tw2 = a.tw + a.f.st[2].twoff;
-    for (int q = t; q < kP4St1 + kP4Lo; q += 1024)
-      st1l[q] = q < 16 ? C2<float>{1.f, 0.f}
-                       : dirw(q < kP4St1 ? tw1[q - 16] : tw2[q - kP4St1], dir);
-  }
-  V Are[O::terms], Aim[O::terms];
-#pragma unroll
-  for (int k = 0; k < O::terms; ++k) {
-    Are[k] = O::load(aop + ((2 * k) * 64 + lane) * 8);
-    Aim[k] = O::load(aop + ((2 * k + 1) * 64 + lane) * 8);
-  }
   const __amdgpu_buffer_rsrc_t rin = bufio::make_rsrc(a.in, in_bytes);
   const __amdgpu_buffer_rsrc_t rout = bufio::make_rsrc(a.out, out_bytes);
   const uint32_t es_in = uint32_t(a.esi) * 4u, es_out = uint32_t(a.eso) * 4u;
@@ -749,6 +739,22 @@ __global__ void __launch_bounds__(1024, 1)
         in[g][i] = bufio::bload(rin, vo, uint32_t(16 * g + NR * i) * es_in);
   };
   load_tile(0, tile_of(0));
+  // the tables' and operands' loads queue behind the first tile's, so both
+  // are in flight together (the first stage waits for all of them anyway)
+  {
+    const C2<float> *tw1 = a.tw + a.f.st[1].twoff;
+    const C2<float> *tw2 = a.tw + a.f.st[2].twoff;
+    for (int q = t; q < kP4St1 + kP4Lo; q += 1024)
+      st1l[q] = q < 16 ? C2<float>{1.f, 0.f}
+                       : dirw(q < kP4St1 ? tw1[q - 16] : tw2[q - kP4St1], dir);
+  }
+  V Are[O::terms], Aim[O::terms];
+#pragma unroll
+  for (int k = 0; k < O::terms; ++k) {
+    Are[k] = O::load(aop + ((2 * k) * 64 + lane) * 8);
+    Aim[k] = O::load(aop + ((2 * k + 1) * 64 + lane) * 8);
+  }
+  __syncthreads();
 
   for (int64_t k = 0;; ++k) {
     int z;  // opaque zero: keeps the LDS twiddle reads in the loop

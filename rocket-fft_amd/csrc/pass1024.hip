@@ -98,16 +98,6 @@ __global__ void __launch_bounds__(1024, 1)
   C2<float> *st1l = lds + kL * kP;
   C2<float> *st2l = st1l + kSt1;
   C2<float> *twl = st2l + kSt2;
-  {
-    const C2<float> *tw1 = a.tw + a.f.st[1].twoff;
-    const C2<float> *tw2 = a.tw + a.f.st[2].twoff;
-    for (int q = t; q < kSt1 + kSt2; q += 1024)
-      st1l[q] = q < 16 ? C2<float>{1.f, 0.f} : dirw(q < kSt1 ? tw1[q - 16] : tw2[q - kSt1], dir);
-  }
-  if (PASS_A) {
-    const int nt = a.twlds_lo + a.twlds_hi;
-    for (int q = t; q < nt; q += 1024) twl[q] = q < a.twlds_lo ? a.t0[q] : a.t1[q - a.twlds_lo];
-  }
   const __amdgpu_buffer_rsrc_t rin = make_rsrc(a.in, in_bytes);
   const __amdgpu_buffer_rsrc_t rout = make_rsrc(a.out, out_bytes);
   const uint32_t es_in = uint32_t(a.esi) * 4u, es_out = uint32_t(a.eso) * 4u;
@@ -133,6 +123,19 @@ __global__ void __launch_bounds__(1024, 1)
     }
   };
   load_tile(0, tile_of(0));
+  // the tables' loads queue behind the first tile's, so both are in flight
+  // together (the first stage waits for all of them anyway)
+  {
+    const C2<float> *tw1 = a.tw + a.f.st[1].twoff;
+    const C2<float> *tw2 = a.tw + a.f.st[2].twoff;
+    for (int q = t; q < kSt1 + kSt2; q += 1024)
+      st1l[q] = q < 16 ? C2<float>{1.f, 0.f} : dirw(q < kSt1 ? tw1[q - 16] : tw2[q - kSt1], dir);
+  }
+  if (PASS_A) {
+    const int nt = a.twlds_lo + a.twlds_hi;
+    for (int q = t; q < nt; q += 1024) twl[q] = q < a.twlds_lo ? a.t0[q] : a.t1[q - a.twlds_lo];
+  }
+  __syncthreads();
 
   for (int64_t k = 0;; ++k) {
     // an opaque zero in the stage-twiddle indices keeps their reads in the

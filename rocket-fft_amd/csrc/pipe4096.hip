@@ -75,12 +75,6 @@ __global__ void __launch_bounds__(1024, 1)
   // table.
   const float dir = a.dir;
   C2<float> *st1l = lds + kL * kP;
-  {
-    const C2<float> *tw1 = a.tw + a.f.st[1].twoff;
-    for (int q = t; q < kSt1; q += 1024)
-      st1l[q] = q < 16 ? C2<float>{1.f, 0.f} : dirw(tw1[q - 16], dir);
-  }
-  const C2<float> aj = dirw(a.tw[a.f.st[2].twoff + (i & 15) * (i >> 4)], dir);
   const __amdgpu_buffer_rsrc_t rin = make_rsrc(a.in, in_bytes);
   const __amdgpu_buffer_rsrc_t rout = make_rsrc(a.out, out_bytes);
   const uint32_t es_in = uint32_t(a.esi) * 4u, es_out = uint32_t(a.eso) * 4u;
@@ -106,6 +100,15 @@ __global__ void __launch_bounds__(1024, 1)
     }
   };
   load_tile(0, tile_of(0), 0);
+  // the tables' loads queue behind the first tile's, so both are in flight
+  // together (the first stage waits for all of them anyway)
+  {
+    const C2<float> *tw1 = a.tw + a.f.st[1].twoff;
+    for (int q = t; q < kSt1; q += 1024)
+      st1l[q] = q < 16 ? C2<float>{1.f, 0.f} : dirw(tw1[q - 16], dir);
+  }
+  const C2<float> aj = dirw(a.tw[a.f.st[2].twoff + (i & 15) * (i >> 4)], dir);
+  __syncthreads();
 
   for (int64_t k = 0;; ++k) {
     // an opaque zero in the twiddle and rotated-element indices keeps their
