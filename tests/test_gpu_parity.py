@@ -135,6 +135,49 @@ def test_c2c_4096_pipelined(lines):
     check("c2c", x, x, np.array([1]), False, 1.0 / n)
 
 
+def _labels_of(fn):
+    import torch
+    R.kernel_timing_report()
+    R.kernel_timing(True)
+    try:
+        fn()
+        torch.cuda.synchronize()
+    finally:
+        R.kernel_timing(False)
+    return R.kernel_timing_report()
+
+
+@pytest.mark.parametrize("case", ["step2", "batch2d"])
+def test_pipelined_kernels_on_views(case):
+    """The persistent pipelined kernels on non-contiguous device views:
+    lines of 4096 and of 2^20 with an element step of 2 (every other element
+    of longer rows), and lines of 4096 under two unmergeable batch
+    dimensions (a transposed batch); against the oracle."""
+    import torch
+    rs = np.random.RandomState(11)
+    if case == "step2":
+        shapes = [((2048, 8192), "k_pipe4096 c2c n=4096"), ((3, 1 << 21), "k_pass1024 passA")]
+    else:
+        shapes = [((1024, 2, 4096), "k_pipe4096 c2c n=4096")]
+    for shape, family in shapes:
+        base = crandn(rs, shape, np.complex64)
+        tb = torch.from_numpy(base).cuda()
+        if case == "step2":
+            view, hv = tb[:, ::2], base[:, ::2]
+        else:
+            view, hv = tb.transpose(0, 1), base.transpose(1, 0, 2)
+        ax = np.array([view.dim() - 1])
+        print(f"case pipelined view {case} base={shape} view={tuple(view.shape)} "
+              f"strides={view.stride()}", flush=True)
+        out = torch.zeros(view.shape, dtype=torch.complex64, device="cuda")
+        labels = _labels_of(lambda: R.c2c(view, out, ax, True, 1.0))
+        assert any(k.startswith(family) for k in labels), sorted(labels)
+        want = oracle_out("c2c", np.ascontiguousarray(hv), np.ascontiguousarray(hv), ax, True, 1.0)
+        e = rel_l2(out.cpu().numpy(), want)
+        assert e <= 1e-6, e
+        assert torch.equal(tb, torch.from_numpy(base).cuda()), "input modified"
+
+
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 @pytest.mark.parametrize("n", [8193, 32749, 65537])
 def test_bluestein_pow2_rows(dtype, n):
