@@ -45,6 +45,10 @@ constexpr int OP_PRE = 9;   // one line: real / half-complex input -> complex wo
 constexpr int OP_POST = 10; // one line: complex work line -> real / half-complex output
 // elements of work a unit gathers from consecutive tiles of one pass
 constexpr int64_t kUnitElems = 4096;
+// elements of a pre / post line per tile: the elementwise real-transform
+// steps are cut into tiles like the line passes, so that a long line's
+// pre / post step spreads over many workgroups instead of one
+constexpr int64_t kPrePostChunk = 2048;
 
 struct PassDev {
   int op;
@@ -56,22 +60,27 @@ struct UnitDev {  // tiles tile0 .. tile0 + ntiles - 1 of pass `pass`
   int ntiles;
 };
 
+// elements [tile C, (tile + 1) C) of a pre / post line (C = kPrePostChunk)
 template <typename T>
-__device__ void pre_line(const LineArgs<T> &a) {
+__device__ void pre_line(const LineArgs<T> &a, int tile) {
   C2<T> *t = reinterpret_cast<C2<T> *>(a.out);
   const int mode = a.sub & 0xff;
-  for (int64_t j = threadIdx.x; j < a.f.n; j += blockDim.x)
+  const int64_t j0 = int64_t(tile) * kPrePostChunk;
+  const int64_t j1 = min(j0 + kPrePostChunk, int64_t(a.f.n));
+  for (int64_t j = j0 + threadIdx.x; j < j1; j += blockDim.x)
     t[j] = pre_elem<T>(mode, a.in, a.esi, a.bign, a.f.n, j, false, (a.sub & R2R_CONJ) != 0,
                        false);
 }
 
 template <typename T>
-__device__ void post_line(const LineArgs<T> &a) {
+__device__ void post_line(const LineArgs<T> &a, int tile) {
   const C2<T> *t = reinterpret_cast<const C2<T> *>(a.in);
   const int mode = (a.sub >> 8) & 0xff;
   const int64_t n = a.bign;
+  const int64_t k0 = int64_t(tile) * kPrePostChunk;
   if (mode == kPostComplexHalf || mode == kPostHalfSpectrum) {
-    for (int64_t k = threadIdx.x; k <= n / 2; k += blockDim.x) {
+    const int64_t k1 = min(k0 + kPrePostChunk, n / 2 + 1);
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
       C2<T> v = mode == kPostComplexHalf
                     ? t[k]
                     : half_spectrum_bin<T>([&](int64_t q) { return t[q]; }, n, k);
@@ -80,7 +89,8 @@ __device__ void post_line(const LineArgs<T> &a) {
       a.out[k * a.eso + 1] = a.fct * v.y;
     }
   } else {
-    for (int64_t k = threadIdx.x; k < n; k += blockDim.x)
+    const int64_t k1 = min(k0 + kPrePostChunk, n);
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x)
       a.out[k * a.eso] =
           a.fct * post_elem<T>(mode, [&](int64_t q) { return t[q]; }, n, k, false, false);
   }
@@ -116,8 +126,8 @@ __global__ void __launch_bounds__(kThreads) k_ragged(const UnitDev *__restrict__
         case OP_R2C_ODD: line_body<T, OP_R2C_ODD, false, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
         case OP_C2R_EVEN: line_body<T, OP_C2R_EVEN, true, true>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
         case OP_C2R_ODD: line_body<T, OP_C2R_ODD, true, false>(a, t, smem_raw, s_oi, s_oo, s_aux); break;
-        case OP_PRE: pre_line<T>(a); break;
-        default: post_line<T>(a); break;
+        case OP_PRE: pre_line<T>(a, t); break;
+        default: post_line<T>(a, t); break;
       }
       __syncthreads();  // tile boundary: LDS and per-line slots are reused
     }
@@ -264,13 +274,21 @@ std::unique_ptr<Built> build(const std::vector<rfh_job_t> &jobs, bool dbl, size_
                          " needs the multi-pass Bluestein form, which the persistent kernel "
                          "does not run");
       LineArgs<T> a = make_line_args<T>(pl.ps, pl.g);
+      int tiles = int((pl.g.nlines + a.L - 1) / a.L);
       if (pl.ps.op == OP_PRE || pl.ps.op == OP_POST) {
         a.f.n = int(cplan[i]->n);  // work-line length m
         a.L = 1;
+        // outputs of the step: the work line, or the half spectrum / real line
+        const int post_mode = (a.sub >> 8) & 0xff;
+        const int64_t count =
+            pl.ps.op == OP_PRE ? int64_t(a.f.n)
+            : (post_mode == kPostComplexHalf || post_mode == kPostHalfSpectrum) ? a.bign / 2 + 1
+                                                                                 : a.bign;
+        tiles = int((count + kPrePostChunk - 1) / kPrePostChunk);
       } else {
         lds_need = std::max<int>(lds_need, int(2 * a.L * a.ld * ce));
       }
-      pd.push_back(PassDev{pl.ps.op, int((pl.g.nlines + a.L - 1) / a.L)});
+      pd.push_back(PassDev{pl.ps.op, tiles});
       la.push_back(a);
     }
     cost.push_back({job_cost(per[i]), int(i)});
@@ -290,7 +308,9 @@ std::unique_ptr<Built> build(const std::vector<rfh_job_t> &jobs, bool dbl, size_
       const int pid = first[i] + lv;
       const LineArgs<T> &a = la[size_t(pid)];
       // a unit holds enough tiles for about kUnitElems elements of work
-      const int64_t per_tile = int64_t(std::max(1, a.L)) * std::max(1, a.f.n);
+      const bool pp = pd[size_t(pid)].op == OP_PRE || pd[size_t(pid)].op == OP_POST;
+      const int64_t per_tile =
+          pp ? kPrePostChunk : int64_t(std::max(1, a.L)) * std::max(1, a.f.n);
       const int tpu = int(std::max<int64_t>(1, kUnitElems / per_tile));
       for (int t0 = 0; t0 < pd[size_t(pid)].tiles; t0 += tpu)
         units.push_back(UnitDev{pid, t0, std::min(tpu, pd[size_t(pid)].tiles - t0)});
