@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 
 #include "device.hpp"
@@ -51,13 +52,18 @@ inline int64_t line_span(const dev::LineArgs<float> &a, int n, bool in) {
   return s;
 }
 
-// compute units of the current device (256 when the query fails)
+// compute units of the current device (256 when the query fails), queried
+// once per device
 inline int device_cus() {
+  constexpr int kDevs = 64;
+  static std::atomic<int> cache[kDevs];
   int dev = 0, v = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (dev >= 0 && dev < kDevs && (v = cache[dev].load(std::memory_order_relaxed)) > 0) return v;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       v <= 0)
     return 256;
+  if (dev >= 0 && dev < kDevs) cache[dev].store(v, std::memory_order_relaxed);
   return v;
 }
 
