@@ -15,9 +15,11 @@
 // twiddles come from LDS or registers, so that no global load runs between
 // a prefetch and its use (the memory counter completes in order), and each
 // stage's twiddles are applied as the stage before writes its outputs:
-// stage 1's W_256^(r k) from the plan's table in LDS; stage 2's W_4096^e
-// split as W_4096^(e mod 16 * (e / 16 mod 16)) (one register pair per
-// thread, from the plan's table) times W_256^(...) from the same LDS table.
+// stage 1's W_256^(r k) from the plan's table in LDS; stage 2's
+// W_4096^(j2 r2) on stage 1's output q of butterfly j (j2 = j mod 16 + 16 q,
+// r2 = j / 16) as A_j W_256^(q (j / 16)), A_j = W_4096^((j mod 16)(j / 16))
+// held in one register pair per thread (from the plan's table) and W_256
+// from the same LDS table.
 #include <algorithm>
 #include <mutex>
 #include <set>
