@@ -80,3 +80,48 @@ def test_growth_during_capture_is_an_error():
     torch.cuda.synchronize()
     want = np.fft.fft(x_large.cpu().numpy().astype(np.complex128), axis=1)
     assert rel_l2(y_large.cpu().numpy(), want) <= 1e-6
+
+
+@pytest.mark.parametrize("mode", ["fp32", "mfma_fp16"])
+def test_graph_replays_pipelined_kernels(mode):
+    """The persistent pipelined kernels (lines of 4096 and the 2^20 passes,
+    in the FP32 and the matrix-core form) captured in one graph: every
+    replay, with eager transforms and torch work on the same stream in
+    between, equals the eager result bit for bit."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(31)
+    xa = torch.randn((2048, 4096), dtype=torch.complex64, device="cuda", generator=g)
+    xb = torch.randn((3, 1 << 20), dtype=torch.complex64, device="cuda", generator=g)
+    other = torch.randn((512, 4096), dtype=torch.complex64, device="cuda", generator=g)
+    ya, yb, yo = torch.empty_like(xa), torch.empty_like(xb), torch.empty_like(other)
+    s, hip, h = fresh_stream()
+    ax = np.array([1])
+    prev = R.set_precision(mode)
+    try:
+        print(f"case graph replays of the pipelined kernels, mode {mode}", flush=True)
+        with torch.cuda.stream(s):
+            R.c2c(xa, ya, ax, True, 1.0, 1)
+            R.c2c(xb, yb, ax, False, 1.0 / (1 << 20), 1)
+        torch.cuda.synchronize()
+        ea, eb = ya.clone(), yb.clone()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=s):
+            R.c2c(xa, ya, ax, True, 1.0, 1)
+            R.c2c(xb, yb, ax, False, 1.0 / (1 << 20), 1)
+        torch.cuda.synchronize()
+        for rep in range(3):
+            ya.zero_()
+            yb.zero_()
+            with torch.cuda.stream(s):
+                R.c2c(other, yo, ax, True, 1.0, 1)  # eager work between replays
+                yo.mul_(0.5)
+            torch.cuda.synchronize()
+            graph.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(ya, ea), f"replay {rep}: lines of 4096 differ"
+            assert torch.equal(yb, eb), f"replay {rep}: 2^20 lines differ"
+    finally:
+        R.set_precision(prev)
+    want = np.fft.fft(xa[:8].cpu().numpy().astype(np.complex128), axis=1)
+    assert rel_l2(ea[:8].cpu().numpy(), want) <= 1e-6
