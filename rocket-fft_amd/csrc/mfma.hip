@@ -315,166 +315,8 @@ __global__ void __launch_bounds__(256, 4) k_fft4096_mfma(const LineArgs<float> a
 // The radix-4 stage and the stores use the interleaved mapping in both passes
 // (thread t: line t % 16), so each store instruction writes 128-byte
 // segments of 16 adjacent lines.
-template <int MODE, bool PASS_A>
-__global__ void __launch_bounds__(1024, 1) k_pass1024_mfma(const LineArgs<float> a,
-                                                          const float *__restrict__ aop) {
-  using O = Ops<MODE>;
-  typedef typename O::V V;
-  constexpr int N = 1024, L = 16, P = N + (N >> 4) + 1;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  C2<float> *lds = reinterpret_cast<C2<float> *>(smem_raw);
-  __shared__ int64_t s_oi[L], s_oo[L];
-  __shared__ int s_aux[L];
-  __shared__ float s_max[16][L];
-  const int t = threadIdx.x, wave = t >> 6, lane = t & 63, col = lane & 15, kg = lane >> 4;
-  const int r0 = (kg & 1) * 8 + (kg >= 2 ? 4 : 0);
-  const int64_t b0 = int64_t(xcd_tile()) * L;
-  const int nl = int(min(int64_t(L), a.nlines - b0));
-  if (t < L) {  // offsets of the tile's lines (zeros for missing lines, never used)
-    const LineOffsets o = group_slot(b0, t, L, a.nlines, a.nb, a.bdim, a.bsi, a.bso, a.auxdim);
-    s_oi[t] = o.oi;
-    s_oo[t] = o.oo;
-    s_aux[t] = o.aux;
-  }
-  // pass A: the four-step twiddle tables (lo / hi halves) behind the lines
-  // in LDS when the launch reserved room for them
-  C2<float> *twl = reinterpret_cast<C2<float> *>(smem_raw + size_t(L) * P * sizeof(C2<float>));
-  const bool tw_lds = PASS_A && a.twlds_lo > 0;
-  if (tw_lds) {
-    const int nt = a.twlds_lo + a.twlds_hi;
-    for (int q = t; q < nt; q += 1024) twl[q] = q < a.twlds_lo ? a.t0[q] : a.t1[q - a.twlds_lo];
-  }
-  V Are[O::terms], Aim[O::terms];
-#pragma unroll
-  for (int k = 0; k < O::terms; ++k) {
-    Are[k] = O::load(aop + ((2 * k) * 64 + lane) * 8);
-    Aim[k] = O::load(aop + ((2 * k + 1) * 64 + lane) * 8);
-  }
-  __syncthreads();
-  const float dir = a.dir;
-  const int ml = PASS_A ? col : wave;  // line of this lane in the radix-16 stages
-  const bool mvalid = ml < nl;
-  C2<float> *line = lds + ml * P;
-  // butterfly of this lane in group g of a radix-16 stage
-  auto bj = [&](int g) { return PASS_A ? wave * 4 + g : g * 16 + col; };
-
-  // ---- stage 0 (radix 16, NS = 1): inputs from memory
-  C2<float> in[4][4];
-  {
-    const float *gin = a.in + s_oi[ml];
-#pragma unroll
-    for (int g = 0; g < 4; ++g)
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        in[g][i] = mvalid ? ldc<true>(gin + int64_t(bj(g) + 64 * (r0 + i)) * a.esi)
-                          : C2<float>{0.f, 0.f};
-  }
-  int ex = 0;  // FP16: exponent of the line's largest input
-  if constexpr (MODE == MODE_F16) {
-    float m = 0.f;
-#pragma unroll
-    for (int g = 0; g < 4; ++g)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) m = fmaxf(m, fmaxf(fabsf(in[g][i].x), fabsf(in[g][i].y)));
-    m = fmaxf(m, __shfl_xor(m, 16));
-    m = fmaxf(m, __shfl_xor(m, 32));
-    if constexpr (PASS_A) {  // line = column: reduce over the 16 waves
-      if (kg == 0) s_max[wave][col] = m;
-      __syncthreads();
-      m = 0.f;
-#pragma unroll
-      for (int w = 0; w < 16; ++w) m = fmaxf(m, s_max[w][col]);
-    } else {  // line = wave
-#pragma unroll
-      for (int o = 1; o < 16; o <<= 1) m = fmaxf(m, __shfl_xor(m, o));
-    }
-    (void)frexpf(m, &ex);
-    if (m == 0.f) ex = 10;
-  }
-  // FP16 operands of stage s are scaled by 2^(10 - ex - 4 s): stage s inputs
-  // are at most 16^s times the line's largest input (exact, undone after)
-  const float sc0 = MODE == MODE_F16 ? ldexpf(1.f, 10 - ex) : 1.f;
-  const float inv0 = MODE == MODE_F16 ? ldexpf(1.f, ex - 10) : 1.f;
-  {
-    f4 dre[4], dim[4];
-#pragma unroll
-    for (int g = 0; g < 4; ++g) dft16_mma<MODE>(in[g], sc0, Are, Aim, dre[g], dim[g]);
-#pragma unroll
-    for (int g = 0; g < 4; ++g)
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        line[padf(bj(g) * 16 + 4 * kg + q)] = C2<float>{dre[g][q] * inv0, dim[g][q] * inv0};
-  }
-  __syncthreads();
-  // ---- stage 1 (radix 16, NS = 16)
-  {
-    const C2<float> *tw = a.tw + a.f.st[1].twoff;
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const int j = bj(g);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = r0 + i;
-        C2<float> v = line[padf(j + 64 * r)];
-        if (r > 0) v = cmul(v, dirw(tw[(r - 1) * 16 + (j & 15)], dir));
-        in[g][i] = v;
-      }
-    }
-    const float sc1 = MODE == MODE_F16 ? ldexpf(1.f, 6 - ex) : 1.f;
-    const float inv1 = MODE == MODE_F16 ? ldexpf(1.f, ex - 6) : 1.f;
-    f4 dre[4], dim[4];
-#pragma unroll
-    for (int g = 0; g < 4; ++g) dft16_mma<MODE>(in[g], sc1, Are, Aim, dre[g], dim[g]);
-    __syncthreads();  // every read of stage 1 is done
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const int j = bj(g);
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        line[padf((j >> 4) * 256 + (j & 15) + (4 * kg + q) * 16)] =
-            C2<float>{dre[g][q] * inv1, dim[g][q] * inv1};
-    }
-  }
-  __syncthreads();
-  // ---- stage 2 (radix 4, NS = 256) on the VALU, stores of natural-order bins
-  const int l2 = t & 15, i2 = t >> 4;
-  if (l2 < nl) {
-    const C2<float> *ln = lds + l2 * P;
-    const C2<float> *tw = a.tw + a.f.st[2].twoff;
-    float *gout = a.out + s_oo[l2];
-    const int aux = s_aux[l2];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int j = i2 + 64 * q;
-      C2<float> v[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) v[r] = ln[padf(j + 256 * r)];
-#pragma unroll
-      for (int r = 1; r < 4; ++r) v[r] = cmul(v[r], dirw(tw[(r - 1) * 256 + j], dir));
-      dft4(v, dir);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int k = j + 256 * r;
-        if constexpr (PASS_A) {
-          const int64_t e = int64_t(k) * aux;
-          C2<float> w;
-          if (tw_lds) {
-            const int64_t mask = (int64_t(1) << a.shift) - 1;
-            w = cmul(twl[e & mask], twl[a.twlds_lo + (e >> a.shift)]);
-          } else {
-            w = big_twiddle(a, e);
-          }
-          stc<true>(gout + int64_t(k) * a.eso, cmul(v[r], dirw(w, dir)));
-        } else {
-          stc<true>(gout + int64_t(k) * a.eso, a.fct * v[r]);
-        }
-      }
-    }
-  }
-}
-
-// The same passes, persistent and software-pipelined like the FP32 form
-// (pass1024.hip): one 1024-thread workgroup per CU walks its 16-line tiles
+// Persistent and software-pipelined like the FP32 form (pass1024.hip): one
+// 1024-thread workgroup per CU walks its 16-line tiles
 // and issues the next tile's loads (buffer instructions, scalar per-element
 // offsets) right after the current tile's first stage; no global load runs
 // between a prefetch and its use (the memory counter completes in order), so
@@ -1004,36 +846,8 @@ const char *launch_pass1024(int mode, int op, const dev::LineArgs<float> &a, boo
   if (!pass_a && !pass_b) return nullptr;
   if (a.f.nst != 3 || a.f.st[0].radix != 16 || a.f.st[1].radix != 16 || a.f.st[2].radix != 4)
     return nullptr;
-  const void *fn = nullptr;
-#define RFH_PASS_FN(M)                                                                     \
-  fn = pass_a ? reinterpret_cast<const void *>(&k_pass1024_mfma<M, true>)                 \
-              : reinterpret_cast<const void *>(&k_pass1024_mfma<M, false>)
-  switch (mode) {
-    case MODE_F16: RFH_PASS_FN(MODE_F16); break;
-    case MODE_BF16_3: RFH_PASS_FN(MODE_BF16_3); break;
-    default: RFH_PASS_FN(MODE_BF16_2); break;
-  }
-#undef RFH_PASS_FN
   if (a.sub == 0 && launch_pass1024_pipe(mode, pass_a, a, s)) return "k_pass1024_mfma_pipe";
-  size_t lds = size_t(16) * (1024 + 64 + 1) * sizeof(C2<float>);
-  dev::LineArgs<float> la = a;
-  la.twlds_lo = la.twlds_hi = 0;
-  if (pass_a) {  // four-step twiddle tables in LDS when they fit beside the tile
-    const int64_t big = int64_t(1024) * a.other;
-    const int64_t lo_n = int64_t(1) << a.shift, hi_n = (big + lo_n - 1) / lo_n;
-    const size_t extra = size_t(lo_n + hi_n) * sizeof(C2<float>);
-    if (lds + extra + 2048 <= lds_limit()) {
-      la.twlds_lo = int(lo_n);
-      la.twlds_hi = int(hi_n);
-      lds += extra;
-    }
-  }
-  set_lds_attr(fn);
-  const float *aop = operands(mode, a.dir < 0 ? -1 : 1);
-  const unsigned grid = unsigned((a.nlines + 15) / 16);
-  void *args[] = {&la, &aop};
-  HIP_CHECK(hipLaunchKernel(fn, dim3(grid), dim3(1024), args, lds, s));
-  return "k_pass1024_mfma";
+  return nullptr;  // spans or tables beyond the pipelined form: the FP32 kernels run it
 }
 
 // Contiguous complex64 lines of 4096 on the persistent pipelined kernel:

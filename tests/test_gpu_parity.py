@@ -367,6 +367,20 @@ def test_two_pass_pow2_fast(dtype, n):
     check("c2c", x, x, np.array([1]), False, 1.0 / n)
 
 
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_two_pass_2p20_generic(dtype):
+    """The 2^20 four-step passes with the specialised and pipelined kernels
+    switched off: the generic line kernel runs both passes."""
+    R.set_fast_path(False)
+    try:
+        rs = np.random.RandomState(12)
+        x = crandn(rs, (2, 1 << 20), dtype)
+        check("c2c", x, x, np.array([1]), True, 1.0)
+        check("c2c", x, x, np.array([1]), False, 1.0 / (1 << 20))
+    finally:
+        R.set_fast_path(True)
+
+
 @pytest.mark.parametrize("kind,type,n", [("dct", 2, 4099), ("dst", 3, 4099), ("dct", 4, 8192),
                                          ("dct", 1, 6000), ("dst", 1, 5000), ("dct", 3, 12000),
                                          ("dst", 2, 12000)])
