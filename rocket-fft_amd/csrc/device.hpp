@@ -847,9 +847,10 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
         v = ldc<VIN>(base + rot_index(j, a.rot_in, a.nin) * a.esi);
       }
     } else if constexpr (OP == OP_BLUE) {
-      if (j < a.bign) {
+      if (j < a.bign) {  // element (j + rot_in) mod n, times the chirp at j
         const C2<T> c = a.t0[j];
-        v = cmul(ldc<VIN>(base + j * a.esi), conjf ? c : conjc(c));
+        v = cmul(ldc<VIN>(base + rot_index(j, a.rot_in, int(a.bign)) * a.esi),
+                 conjf ? c : conjc(c));
       } else {
         v = C2<T>{T(0), T(0)};
       }
@@ -926,9 +927,10 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
       } else {
         stc<VOUT>(base + rot_index(k, a.rot_out, a.nout) * a.eso, a.fct * v);
       }
-    } else if constexpr (OP == OP_BLUE) {
+    } else if constexpr (OP == OP_BLUE) {  // bin k to (k + rot_out) mod n
       const C2<T> c = a.t0[k];
-      stc<VOUT>(base + k * a.eso, a.fct * cmul(v, conjf ? c : conjc(c)));
+      stc<VOUT>(base + rot_index(k, a.rot_out, a.nout) * a.eso,
+                a.fct * cmul(v, conjf ? c : conjc(c)));
     } else if constexpr (OP == OP_R2C_ODD) {
       if (conjf) v.y = -v.y;
       stc<VOUT>(base + k * a.eso, a.fct * v);

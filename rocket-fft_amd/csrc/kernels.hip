@@ -640,7 +640,7 @@ bool c2c_rotation_fusable(const C2CPlan &p, int64_t rot_in, int64_t rot_out) {
   if (rot_in < 0 || rot_out < 0 || uint64_t(rot_in) >= p.n || uint64_t(rot_out) >= p.n)
     return false;
   if (!rot_in && !rot_out) return true;
-  if (p.kind == kDirect) return true;
+  if (p.kind == kDirect || p.kind == kBlueDirect) return true;
   if (p.kind == kTwoPass)
     return uint64_t(rot_in) % p.n2 == 0 && uint64_t(rot_out) % p.n1 == 0;
   return false;

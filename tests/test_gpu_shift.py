@@ -8,9 +8,10 @@ complex64, 1e-13 complex128).  Covers one-pass lengths on the specialised
 power-of-two kernel (64, 1024, 4096) and the generic kernel (1000, odd 243,
 whose fftshift is not a half turn), four-step lengths (2^16, 2^20 complex64),
 strided and transposed arrays, in place, several axes with mixed signs,
-lengths whose plan cannot fuse a shift (Bluestein 4099: nothing written,
-front-end falls back), and the matrix-core precision mode (shifted lines run
-on the FP32 kernels)."""
+Bluestein lengths whose transform is one LDS pass (4099 complex64, 1031
+complex128), lengths whose plan cannot fuse a shift (Bluestein 65537 over
+three passes: nothing written, front-end falls back), and the matrix-core
+precision mode (shifted lines run on the FP32 kernels)."""
 import numpy as np
 import pytest
 
@@ -100,13 +101,28 @@ def test_shift_several_axes_strided_inplace(dtype):
     assert np.array_equal(out.cpu().numpy(), want)
 
 
+@pytest.mark.parametrize("dtype,n", [(np.complex64, 4099), (np.complex128, 1031)])
+def test_fftshift_bluestein_one_pass(dtype, n):
+    """Bluestein lengths whose whole transform is one LDS pass (chirp, FFT,
+    kernel product, FFT, chirp): the rolls rotate its loads and stores."""
+    kind = R.gpu_schedule(n, dtype == np.complex128)[0]
+    print(f"case bluestein one-pass shift n={n} {dtype.__name__} kind={kind}", flush=True)
+    assert kind == 1
+    rs = np.random.RandomState(n)
+    x = crandn(rs, (3, n), dtype)
+    check_case(x, [1], True, 1.0, [-(n // 2)], [n // 2])
+    check_case(x, [1], False, 1.0 / n, [n // 2], [-(n // 2)])
+    check_case(x, [1], True, 1.0, [7], None)
+
+
 def test_not_fusable_writes_nothing_and_frontend_falls_back():
+    """65537: Bluestein over three passes, whose rolls are not fused."""
     import torch
     rs = np.random.RandomState(4)
-    x = crandn(rs, (4, 4099), np.complex64)
+    x = crandn(rs, (4, 65537), np.complex64)
     t = to_gpu(x)
     out = torch.full_like(t, 7.0)
-    assert not R.c2c_shifted(t, out, np.array([1]), True, 1.0, [-2049], [2049])
+    assert not R.c2c_shifted(t, out, np.array([1]), True, 1.0, [-32768], [32768])
     torch.cuda.synchronize()
     assert bool((out == 7.0).all())
     got = F.centered_fftn(t, axes=[1]).cpu().numpy()
