@@ -382,6 +382,19 @@ __device__ __forceinline__ int rot_index(int j, int r, int n) {
   return q >= n ? q - n : q;
 }
 
+// Bluestein column passes (chirped loads of pass A, final stores of pass C):
+// element j of column aux is element m = j other + aux of the transform
+// line; with the line rotated by rot it is read from / written to element
+// (m + rot) mod n, which lies (that - aux) line elements (es / other scalars
+// each) from the column's start.  Offset in scalars from the column start.
+__device__ __forceinline__ int64_t blue_col_offset(int j, int64_t m, int rot, int64_t n,
+                                                   int aux, int64_t es, int64_t other) {
+  if (!rot) return int64_t(j) * es;
+  int64_t q = m + rot;
+  if (q >= n) q -= n;
+  return (q - aux) * (es / other);
+}
+
 template <bool V, typename T>
 __device__ __forceinline__ C2<T> ldc(const T *p) {
   if constexpr (V) return *reinterpret_cast<const C2<T> *>(p);
@@ -839,7 +852,9 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
         const int64_t m = int64_t(j) * a.other + s_aux[l];
         if (m < a.bign) {
           const C2<T> c = a.t2[m];
-          v = cmul(ldc<VIN>(base + j * a.esi), conjf ? c : conjc(c));
+          v = cmul(ldc<VIN>(base + blue_col_offset(j, m, a.rot_in, a.bign, s_aux[l], a.esi,
+                                                   a.other)),
+                   conjf ? c : conjc(c));
         } else {
           v = C2<T>{T(0), T(0)};
         }
@@ -905,7 +920,8 @@ __device__ __forceinline__ void line_body(const LineArgs<T> &a, int64_t tile,
         const int64_t m = s_aux[l] + int64_t(k) * a.other;
         if (m < a.bign) {
           const C2<T> c = a.t0[m];
-          stc<VOUT>(base + k * a.eso, a.fct * cmul(v, conjf ? c : conjc(c)));
+          stc<VOUT>(base + blue_col_offset(k, m, a.rot_out, a.bign, s_aux[l], a.eso, a.other),
+                    a.fct * cmul(v, conjf ? c : conjc(c)));
         }
       } else {
         const C2<T> w = dirw(big_twiddle(a, int64_t(k) * s_aux[l]), a.dir);

@@ -108,7 +108,8 @@ __device__ __forceinline__ C2<T> fload(const FCtx<T> &c, int p) {
       const int64_t m = int64_t(p) * a.other + c.aux;
       if (m >= a.bign) return C2<T>{T(0), T(0)};
       const C2<T> ch = a.t2[m];
-      return cmul(ldc<VIN>(c.gin + p * a.esi), (a.sub & SUB_CONJ) ? ch : conjc(ch));
+      const int64_t off = blue_col_offset(p, m, a.rot_in, a.bign, c.aux, a.esi, a.other);
+      return cmul(ldc<VIN>(c.gin + off), (a.sub & SUB_CONJ) ? ch : conjc(ch));
     }
     return ldc<VIN>(c.gin + ((p + a.rot_in) & (a.f.n - 1)) * a.esi);
   } else {  // OP_C2C (power-of-two lengths: the rotation is a mask)
@@ -138,7 +139,8 @@ __device__ __forceinline__ void fstore(const LineArgs<T> &a, const C2<T> *twl, T
       const int64_t m = aux + int64_t(p) * a.other;
       if (m < a.bign) {
         const C2<T> ch = a.t0[m];
-        stc<VOUT>(gout + p * a.eso, a.fct * cmul(v, (a.sub & SUB_CONJ) ? ch : conjc(ch)));
+        stc<VOUT>(gout + blue_col_offset(p, m, a.rot_out, a.bign, aux, a.eso, a.other),
+                  a.fct * cmul(v, (a.sub & SUB_CONJ) ? ch : conjc(ch)));
       }
     } else {
       const C2<T> w = dirw(ftwiddle(a, twl, int64_t(p) * aux), a.dir);

@@ -640,7 +640,7 @@ bool c2c_rotation_fusable(const C2CPlan &p, int64_t rot_in, int64_t rot_out) {
   if (rot_in < 0 || rot_out < 0 || uint64_t(rot_in) >= p.n || uint64_t(rot_out) >= p.n)
     return false;
   if (!rot_in && !rot_out) return true;
-  if (p.kind == kDirect || p.kind == kBlueDirect) return true;
+  if (p.kind == kDirect || p.kind == kBlueDirect || p.kind == kBlueTwoPass) return true;
   if (p.kind == kTwoPass)
     return uint64_t(rot_in) % p.n2 == 0 && uint64_t(rot_out) % p.n1 == 0;
   return false;
@@ -756,7 +756,13 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     a.t2 = p.bk.p;
     a.nin = a.nout = int(n1);
     a.lines = pass_lines(p, n1, lines);
-    out.push_back(PassLaunch{a, pass_a(g, n1, n2, g.in, g.esi, g.bsi, t)});
+    // rotations of the transform line: the chirped loads of pass A read
+    // element (m + rot_in) mod n, the final stores of pass C write bin m to
+    // (m + rot_out) mod n (whole-line element rotations, not column ones)
+    LineSet ga = pass_a(g, n1, n2, g.in, g.esi, g.bsi, t);
+    ga.rot_in = g.rot_in;
+    ga.rot_out = 0;
+    out.push_back(PassLaunch{a, ga});
     PassSpec r;
     r.d = &p.pb;
     r.op = OP_BLUEROW;
@@ -781,7 +787,10 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     c.t0 = p.bk.p;
     c.nin = c.nout = int(n1);
     c.lines = pass_lines(p, n1, lines);
-    out.push_back(PassLaunch{c, pass_c(g, n2, t, g.out, g.eso, g.bso)});
+    LineSet gc = pass_c(g, n2, t, g.out, g.eso, g.bso);
+    gc.rot_in = 0;
+    gc.rot_out = g.rot_out;
+    out.push_back(PassLaunch{c, gc});
   } else {  // kBlueMulti: elementwise chirp and kernel passes around the sub-plan of bn2
     const uint64_t N = p.bn2;
     const Work wl = work_layout(g, N, inter);

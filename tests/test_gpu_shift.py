@@ -9,9 +9,10 @@ power-of-two kernel (64, 1024, 4096) and the generic kernel (1000, odd 243,
 whose fftshift is not a half turn), four-step lengths (2^16, 2^20 complex64),
 strided and transposed arrays, in place, several axes with mixed signs,
 Bluestein lengths whose transform is one LDS pass (4099 complex64, 1031
-complex128), lengths whose plan cannot fuse a shift (Bluestein 65537 over
-three passes: nothing written, front-end falls back), and the matrix-core
-precision mode (shifted lines run on the FP32 kernels)."""
+complex128) or three passes (65537 complex64, 4099 complex128), lengths whose
+plan cannot fuse a shift (the multi-pass 2^21: nothing written, front-end
+falls back), and the matrix-core precision mode (shifted lines run on the
+FP32 kernels)."""
 import numpy as np
 import pytest
 
@@ -115,14 +116,31 @@ def test_fftshift_bluestein_one_pass(dtype, n):
     check_case(x, [1], True, 1.0, [7], None)
 
 
+@pytest.mark.parametrize("dtype,n", [(np.complex64, 65537), (np.complex128, 4099)])
+def test_fftshift_bluestein_three_pass(dtype, n):
+    """Bluestein over three passes (chirped columns, rows, final columns):
+    the rolls rotate pass A's chirped loads and pass C's final stores over
+    the whole line."""
+    kind = R.gpu_schedule(n, dtype == np.complex128)[0]
+    print(f"case bluestein three-pass shift n={n} {dtype.__name__} kind={kind}", flush=True)
+    assert kind == 3
+    rs = np.random.RandomState(n)
+    x = crandn(rs, (3, n), dtype)
+    check_case(x, [1], True, 1.0, [-(n // 2)], [n // 2])
+    check_case(x, [1], False, 1.0 / n, [n // 2], [-(n // 2)])
+    check_case(x, [1], True, 1.0, [1], [n - 1])
+
+
 def test_not_fusable_writes_nothing_and_frontend_falls_back():
-    """65537: Bluestein over three passes, whose rolls are not fused."""
+    """2^21 complex64: a multi-pass plan, whose rolls are not fused."""
     import torch
+    n = 1 << 21
+    assert R.gpu_schedule(n, False)[0] == 4
     rs = np.random.RandomState(4)
-    x = crandn(rs, (4, 65537), np.complex64)
+    x = crandn(rs, (2, n), np.complex64)
     t = to_gpu(x)
     out = torch.full_like(t, 7.0)
-    assert not R.c2c_shifted(t, out, np.array([1]), True, 1.0, [-32768], [32768])
+    assert not R.c2c_shifted(t, out, np.array([1]), True, 1.0, [-(n // 2)], [n // 2])
     torch.cuda.synchronize()
     assert bool((out == 7.0).all())
     got = F.centered_fftn(t, axes=[1]).cpu().numpy()
