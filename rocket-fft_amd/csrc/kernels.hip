@@ -643,6 +643,9 @@ bool c2c_rotation_fusable(const C2CPlan &p, int64_t rot_in, int64_t rot_out) {
   if (p.kind == kDirect || p.kind == kBlueDirect || p.kind == kBlueTwoPass) return true;
   if (p.kind == kTwoPass)
     return uint64_t(rot_in) % p.n2 == 0 && uint64_t(rot_out) % p.n1 == 0;
+  if (p.kind == kMultiPass && p.sub)
+    return uint64_t(rot_in) % p.n2 == 0 && uint64_t(rot_out) % p.n1 == 0 &&
+           c2c_rotation_fusable(*p.sub, 0, rot_out / int64_t(p.n1));
   return false;
 }
 
@@ -732,10 +735,15 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     a.t1 = p.big_hi.p;
     a.nin = a.nout = int(n1);
     a.lines = pass_lines(p, n1, lines);
-    out.push_back(PassLaunch{a, pass_a(g, n1, M, g.in, g.esi, g.bsi, t)});
+    // rotations as in the four-step plan: rot_in = t M rotates each column
+    // by t; rot_out = u n1 rotates the sub-plan's output of each row by u
+    LineSet la = pass_a(g, n1, M, g.in, g.esi, g.bsi, t);
+    la.rot_in = g.rot_in / int64_t(M);
+    out.push_back(PassLaunch{a, la});
     void *rest = static_cast<char *>(tmp) + align256(size_t(g.nlines) * n1 * M * elem);
-    for (const PassLaunch &pl :
-         describe_c2c(*p.sub, multi_rows(g, n1, t), dir, fct, rest, chunk_bytes))
+    LineSet rows = multi_rows(g, n1, t);
+    rows.rot_out = g.rot_out / int64_t(n1);
+    for (const PassLaunch &pl : describe_c2c(*p.sub, rows, dir, fct, rest, chunk_bytes))
       out.push_back(pl);
   } else if (p.kind == kBlueTwoPass) {
     // Bluestein in three passes over one intermediate t (n1 x n2 per line):

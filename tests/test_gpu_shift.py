@@ -9,10 +9,11 @@ power-of-two kernel (64, 1024, 4096) and the generic kernel (1000, odd 243,
 whose fftshift is not a half turn), four-step lengths (2^16, 2^20 complex64),
 strided and transposed arrays, in place, several axes with mixed signs,
 Bluestein lengths whose transform is one LDS pass (4099 complex64, 1031
-complex128) or three passes (65537 complex64, 4099 complex128), lengths whose
-plan cannot fuse a shift (the multi-pass 2^21: nothing written, front-end
-falls back), and the matrix-core precision mode (shifted lines run on the
-FP32 kernels)."""
+complex128) or three passes (65537 complex64, 4099 complex128), multi-pass
+lengths (2^21), lengths whose plan cannot fuse a shift (prime 1000003:
+Bluestein around a multi-pass transform; nothing written, front-end falls
+back), and the matrix-core precision mode (shifted lines run on the FP32
+kernels)."""
 import numpy as np
 import pytest
 
@@ -131,11 +132,27 @@ def test_fftshift_bluestein_three_pass(dtype, n):
     check_case(x, [1], True, 1.0, [1], [n - 1])
 
 
-def test_not_fusable_writes_nothing_and_frontend_falls_back():
-    """2^21 complex64: a multi-pass plan, whose rolls are not fused."""
-    import torch
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_fftshift_multi_pass(dtype):
+    """2^21: a column pass of n1 and the row sub-plan of M = 2^21 / n1; the
+    input roll (a multiple of M) rotates the columns, the output roll (a
+    multiple of n1) the sub-plan's row outputs."""
     n = 1 << 21
-    assert R.gpu_schedule(n, False)[0] == 4
+    kind = R.gpu_schedule(n, dtype == np.complex128)[0]
+    print(f"case multi-pass shift n={n} {dtype.__name__} kind={kind}", flush=True)
+    assert kind == 4
+    rs = np.random.RandomState(21)
+    x = crandn(rs, (2, n), dtype)
+    check_case(x, [1], True, 1.0, [-(n // 2)], [n // 2])
+    check_case(x, [1], False, 1.0 / n, [n // 2], None)
+
+
+def test_not_fusable_writes_nothing_and_frontend_falls_back():
+    """Prime 1000003 complex64: Bluestein around a multi-pass transform,
+    whose rolls are not fused."""
+    import torch
+    n = 1000003
+    assert R.gpu_schedule(n, False)[0] == 5
     rs = np.random.RandomState(4)
     x = crandn(rs, (2, n), np.complex64)
     t = to_gpu(x)
