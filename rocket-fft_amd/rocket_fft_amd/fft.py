@@ -35,6 +35,7 @@ __all__ = [
     "rfftn", "irfftn", "hfft", "ihfft", "fftfreq", "rfftfreq", "fftshift", "ifftshift",
     "dct", "idct", "dst", "idst", "dctn", "idctn", "dstn", "idstn", "next_fast_len",
     "fht", "ifht", "fhtoffset", "roll", "fftn_shifted", "centered_fftn", "centered_ifftn",
+    "rfftn_shifted",
 ]
 
 
@@ -375,8 +376,9 @@ def fftn_shifted(a, s=None, axes=None, norm=None, forward=True, shift_in=None,
     separate calls (overloads.py:1253-1310 for the shifts); here the rolls
     become index rotations in the loads of each axis's first pass and the
     stores of its last pass (rfh_c2c_shifted), with a bit-identical result.
-    Lengths whose plan cannot fuse them (Bluestein, multi-pass) fall back to
-    one separate gather pass on each side."""
+    Lengths whose plan cannot fuse them (Bluestein around a multi-pass
+    transform; four-step rolls that are not whole columns / rows) fall back
+    to one separate gather pass on each side."""
     a = _asarray(a)
     s, axes = _shape_and_axes(a, s, axes)
     rdt, cdt = _dtype_pair(a)
@@ -398,6 +400,49 @@ def fftn_shifted(a, s=None, axes=None, norm=None, forward=True, shift_in=None,
     if so is not None:
         shifts = [0] * x.ndim
         for ax, v in zip(axes, so):
+            shifts[ax] += v
+        out = _roll_gpu(out, shifts)
+    return out
+
+
+def rfftn_shifted(a, s=None, axes=None, norm=None, shift_in=None, shift_out=None):
+    """roll(rfftn(roll(a, shift_in), s, axes, norm), shift_out), the rolls
+    along the leading transformed axes (axes[:-1]; the half-spectrum axis is
+    not rolled).  The real transform of the last axis runs first (it commutes
+    with rolls along the other axes), then the complex passes of the leading
+    axes in place, with the rolls as index rotations of their loads and
+    stores (rfh_c2c_shifted): no extra pass, and a result bit-identical to
+    the composition.  Plans that cannot fuse a roll fall back to a separate
+    gather pass."""
+    a = _asarray(a)
+    if _is_complex(a):
+        raise TypeError(f"unsupported dtype {a.dtype}")
+    s, axes = _shape_and_axes(a, s, axes)
+    rdt, cdt = _dtype_pair(a)
+    x = _pad_crop(a, s, axes, rdt)
+    shape = list(x.shape)
+    shape[axes[-1]] = shape[axes[-1]] // 2 + 1
+    out = _empty(x, shape, cdt)
+    if x.ndim == 0 or min(x.shape) == 0:
+        return out
+    lead = list(axes[:-1])
+    r2c(x, out, _ax([axes[-1]]), True, _fct(x.shape, axes, norm, True), 1)
+    if not lead:
+        return out
+    si = _shift_amounts(shift_in, x.shape, lead)
+    so = _shift_amounts(shift_out, x.shape, lead)
+    if c2c_shifted(out, out, _ax(lead), True, 1.0, si, so, 1):
+        return out
+    y = out
+    if si is not None:
+        shifts = [0] * x.ndim
+        for ax, v in zip(lead, si):
+            shifts[ax] += v
+        y = _roll_gpu(out, shifts)
+    c2c(y, out, _ax(lead), True, 1.0, 1)
+    if so is not None:
+        shifts = [0] * x.ndim
+        for ax, v in zip(lead, so):
             shifts[ax] += v
         out = _roll_gpu(out, shifts)
     return out
