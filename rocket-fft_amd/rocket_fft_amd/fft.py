@@ -35,7 +35,7 @@ __all__ = [
     "rfftn", "irfftn", "hfft", "ihfft", "fftfreq", "rfftfreq", "fftshift", "ifftshift",
     "dct", "idct", "dst", "idst", "dctn", "idctn", "dstn", "idstn", "next_fast_len",
     "fht", "ifht", "fhtoffset", "roll", "fftn_shifted", "centered_fftn", "centered_ifftn",
-    "rfftn_shifted",
+    "rfftn_shifted", "irfftn_shifted",
 ]
 
 
@@ -445,6 +445,54 @@ def rfftn_shifted(a, s=None, axes=None, norm=None, shift_in=None, shift_out=None
         for ax, v in zip(lead, so):
             shifts[ax] += v
         out = _roll_gpu(out, shifts)
+    return out
+
+
+def irfftn_shifted(a, s=None, axes=None, norm=None, shift_in=None, shift_out=None):
+    """roll(irfftn(roll(a, shift_in), s, axes, norm), shift_out), the rolls
+    along the leading transformed axes (axes[:-1]).  The complex passes of
+    the leading axes run first, into a complex temporary, with both rolls as
+    index rotations of their loads and stores (the real pass of the last
+    axis commutes with rolls along the other axes), then the real pass:
+    the same passes as irfftn.  Plans that cannot fuse a roll fall back to
+    a separate gather pass."""
+    a = _asarray(a)
+    s, axes = _shape_and_axes(a, s, axes)
+    rdt, cdt = _dtype_pair(a)
+    xin = _pad_crop(a, s, axes, cdt)
+    shape = list(a.shape)
+    shape[axes[-1]] = (shape[axes[-1]] - 1) * 2
+    if s is not None:  # overloads.py:655-666: s sets the last array axis
+        last = a.ndim - 1
+        for i, ax in enumerate(axes):
+            if ax == last:
+                shape[last] = s[i]
+                break
+    if shape[axes[-1]] < 1:
+        raise ValueError(f"Invalid number of data points ({shape[axes[-1]]}) specified")
+    out = _empty(xin, shape, rdt)
+    fct = _fct(shape, axes, norm, False)
+    lead = list(axes[:-1])
+    if not lead or min(xin.shape) == 0:
+        c2r(xin, out, _ax(axes), False, fct, 1)
+        return out
+    si = _shift_amounts(shift_in, xin.shape, lead)
+    so = _shift_amounts(shift_out, xin.shape, lead)
+    tmp = _empty(xin, xin.shape, cdt)
+    if not c2c_shifted(xin, tmp, _ax(lead), False, 1.0, si, so, 1):
+        y = xin
+        if si is not None:
+            shifts = [0] * xin.ndim
+            for ax, v in zip(lead, si):
+                shifts[ax] += v
+            y = _roll_gpu(xin, shifts)
+        c2c(y, tmp, _ax(lead), False, 1.0, 1)
+        if so is not None:
+            shifts = [0] * xin.ndim
+            for ax, v in zip(lead, so):
+                shifts[ax] += v
+            tmp = _roll_gpu(tmp, shifts)
+    c2r(tmp, out, _ax([axes[-1]]), False, fct, 1)
     return out
 
 

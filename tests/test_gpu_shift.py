@@ -205,6 +205,29 @@ def test_rfftn_shifted_matches_numpy(dtype):
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_irfftn_shifted_matches_numpy(dtype):
+    """irfftn with rolls along the leading axes (the complex passes take
+    them, then the real pass), and the round trip with rfftn_shifted."""
+    rs = np.random.RandomState(10)
+    tol = tol_of(dtype)
+    x = rs.standard_normal((24, 30, 16))
+    spec = np.fft.rfftn(x).astype(dtype)
+    lead = (0, 1)
+    got = F.irfftn_shifted(to_gpu(spec), shift_in=[3, -4], shift_out="fftshift").cpu().numpy()
+    want = np.fft.fftshift(np.fft.irfftn(np.roll(spec.astype(np.complex128), [3, -4], lead)),
+                           axes=lead)
+    assert rel_l2(got, want) <= tol
+    xs = x.astype(np.float32 if dtype == np.complex64 else np.float64)
+    fwd = F.rfftn_shifted(to_gpu(xs), shift_in="ifftshift", shift_out="fftshift")
+    back = F.irfftn_shifted(fwd, shift_in="ifftshift", shift_out="fftshift").cpu().numpy()
+    assert rel_l2(back, xs) <= 2 * tol
+    got = F.irfftn_shifted(spec, s=(24, 30), axes=(0, 2), shift_out=[-7])  # host array
+    want = np.roll(np.fft.irfftn(spec.astype(np.complex128), s=(24, 30), axes=(0, 2)), -7,
+                   axis=0)
+    assert isinstance(got, np.ndarray) and rel_l2(got, want) <= tol
+
+
 def test_shifted_runs_fp32_kernels_in_matrix_core_mode():
     import torch
     rs = np.random.RandomState(6)
