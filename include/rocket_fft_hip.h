@@ -131,11 +131,11 @@ int rfh_kernel_timing_report(char *buf, size_t len);
  * separate calls).  The rolls become index rotations in the loads of each
  * axis's first pass and the stores of its last pass, so no extra pass runs
  * and the result equals the composition bit for bit.  Fusable: lengths whose
- * plan is one LDS pass, Bluestein lengths in one or three passes, and
- * four-step (multi-pass) lengths n1 x n2 whose input roll is a multiple of
- * n2 and output roll a multiple of n1 (every fftshift / ifftshift of an
- * even length with both factors even); Bluestein around a multi-pass
- * transform returns 5 with nothing written, and the caller
+ * plan is one LDS pass, every Bluestein length, and four-step (multi-pass)
+ * lengths n1 x n2 whose input roll is a multiple of n2 and output roll a
+ * multiple of n1 (every fftshift / ifftshift of an even length with both
+ * factors even); other rolls of those lengths return 5 with nothing
+ * written, and the caller
  * runs the rolls and the transform separately.  Complex64 transforms run
  * on the FP32 kernels whatever rfh_set_precision says.  No reference
  * counterpart.  Returns 0, 5 (not fusable), or another error code. */

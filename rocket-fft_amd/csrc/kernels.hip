@@ -202,19 +202,24 @@ __global__ void __launch_bounds__(kThreads) k_blue_elem(const LineArgs<T> a, int
     for (int64_t m = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; m < a.other;
          m += int64_t(gridDim.x) * blockDim.x) {
       C2<T> v = {T(0), T(0)};
+      int64_t mo = m;  // output element
       if (op == OP_BLUE_IN) {
-        if (m < a.bign) {
+        if (m < a.bign) {  // element (m + rot_in) mod n of the line, times the chirp at m
+          int64_t q = m + a.rot_in;
+          if (q >= a.bign) q -= a.bign;
           const C2<T> c = a.t0[m];
-          v = cmul(ldc<false>(x + m * a.esi), cj ? c : conjc(c));
+          v = cmul(ldc<false>(x + q * a.esi), cj ? c : conjc(c));
         }
       } else if (op == OP_BLUE_MUL) {
         const C2<T> w = a.t1[m];
         v = cmul(ldc<false>(x + m * a.esi), cj ? conjc(w) : w);
-      } else {
+      } else {  // bin m, chirped, to (m + rot_out) mod n
         const C2<T> c = a.t0[m];
         v = a.fct * cmul(ldc<false>(x + m * a.esi), cj ? c : conjc(c));
+        mo = m + a.rot_out;
+        if (mo >= a.other) mo -= a.other;
       }
-      stc<false>(y + m * a.eso, v);
+      stc<false>(y + mo * a.eso, v);
     }
   }
 }
@@ -640,7 +645,9 @@ bool c2c_rotation_fusable(const C2CPlan &p, int64_t rot_in, int64_t rot_out) {
   if (rot_in < 0 || rot_out < 0 || uint64_t(rot_in) >= p.n || uint64_t(rot_out) >= p.n)
     return false;
   if (!rot_in && !rot_out) return true;
-  if (p.kind == kDirect || p.kind == kBlueDirect || p.kind == kBlueTwoPass) return true;
+  if (p.kind == kDirect || p.kind == kBlueDirect || p.kind == kBlueTwoPass ||
+      p.kind == kBlueMulti)
+    return true;
   if (p.kind == kTwoPass)
     return uint64_t(rot_in) % p.n2 == 0 && uint64_t(rot_out) % p.n1 == 0;
   if (p.kind == kMultiPass && p.sub)
@@ -810,14 +817,17 @@ std::vector<PassLaunch> describe_c2c(const C2CPlan &p, const LineSet &g, int dir
     for (int d = 0; d < g.nb; ++d) w.bsi[d] = w.bso[d] = wl.sb[d];
     w.esi = w.eso = wl.e;
     w.auxdim = -1;
-    LineSet gin = g;  // user input -> work lines
+    w.rot_in = w.rot_out = 0;  // the work lines are never rotated
+    LineSet gin = g;  // user input -> work lines (rotated loads)
     gin.out = wp;
     for (int d = 0; d < g.nb; ++d) gin.bso[d] = wl.sb[d];
     gin.eso = wl.e;
-    LineSet gout = g;  // work lines -> user output
+    gin.rot_out = 0;
+    LineSet gout = g;  // work lines -> user output (rotated stores)
     gout.in = wp;
     for (int d = 0; d < g.nb; ++d) gout.bsi[d] = wl.sb[d];
     gout.esi = wl.e;
+    gout.rot_in = 0;
     PassSpec e;
     e.d = &p.main;
     e.sub = sub_conj;

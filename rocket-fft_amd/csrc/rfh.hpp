@@ -195,10 +195,9 @@ void launch_c2c(const C2CPlan &p, const LineSet &g, int dir, double fct, hipStre
 size_t c2c_tmp_bytes(const C2CPlan &p, const LineSet &g, size_t chunk_bytes = 0);
 // Whether plan p runs a line set with element rotations rot_in / rot_out
 // (0 <= rot < n) inside its own passes: one-pass plans (direct and
-// Bluestein) and three-pass Bluestein plans always; multi-pass plans like
-// four-step ones when their row sub-plan takes the output rotation; four-step
-// plans when rot_in is a whole number of columns (n2) and rot_out of rows
-// (n1); the other kinds only without rotations.
+// Bluestein) and every Bluestein plan always; four-step plans when rot_in is
+// a whole number of columns (n2) and rot_out of rows (n1); multi-pass plans
+// likewise when their row sub-plan takes the output rotation.
 bool c2c_rotation_fusable(const C2CPlan &p, int64_t rot_in, int64_t rot_out);
 
 // Opt-in per-launch timing of the line kernels (HIP events on the launching

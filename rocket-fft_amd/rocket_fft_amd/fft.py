@@ -376,9 +376,9 @@ def fftn_shifted(a, s=None, axes=None, norm=None, forward=True, shift_in=None,
     separate calls (overloads.py:1253-1310 for the shifts); here the rolls
     become index rotations in the loads of each axis's first pass and the
     stores of its last pass (rfh_c2c_shifted), with a bit-identical result.
-    Lengths whose plan cannot fuse them (Bluestein around a multi-pass
-    transform; four-step rolls that are not whole columns / rows) fall back
-    to one separate gather pass on each side."""
+    Rolls a plan cannot fuse (four-step or multi-pass rolls that are not
+    whole columns / rows) fall back to one separate gather pass on each
+    side."""
     a = _asarray(a)
     s, axes = _shape_and_axes(a, s, axes)
     rdt, cdt = _dtype_pair(a)

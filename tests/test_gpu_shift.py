@@ -9,11 +9,11 @@ power-of-two kernel (64, 1024, 4096) and the generic kernel (1000, odd 243,
 whose fftshift is not a half turn), four-step lengths (2^16, 2^20 complex64),
 strided and transposed arrays, in place, several axes with mixed signs,
 Bluestein lengths whose transform is one LDS pass (4099 complex64, 1031
-complex128) or three passes (65537 complex64, 4099 complex128), multi-pass
-lengths (2^21), lengths whose plan cannot fuse a shift (prime 1000003:
-Bluestein around a multi-pass transform; nothing written, front-end falls
-back), and the matrix-core precision mode (shifted lines run on the FP32
-kernels)."""
+complex128), three passes (65537 complex64, 4099 complex128) or around a
+multi-pass transform (prime 1000003), multi-pass lengths (2^21), a roll the
+plan cannot fuse (one element on 2^20: nothing written, front-end falls
+back), the real-transform front-ends, and the matrix-core precision mode
+(shifted lines run on the FP32 kernels)."""
 import numpy as np
 import pytest
 
@@ -147,21 +147,33 @@ def test_fftshift_multi_pass(dtype):
     check_case(x, [1], False, 1.0 / n, [n // 2], None)
 
 
-def test_not_fusable_writes_nothing_and_frontend_falls_back():
-    """Prime 1000003 complex64: Bluestein around a multi-pass transform,
-    whose rolls are not fused."""
-    import torch
+def test_fftshift_bluestein_multi_pass():
+    """Prime 1000003: Bluestein around a multi-pass transform; the rolls
+    rotate the elementwise chirp passes' loads (input) and stores (output)."""
     n = 1000003
-    assert R.gpu_schedule(n, False)[0] == 5
+    kind = R.gpu_schedule(n, False)[0]
+    print(f"case bluestein multi-pass shift n={n} kind={kind}", flush=True)
+    assert kind == 5
+    rs = np.random.RandomState(7)
+    x = crandn(rs, (2, n), np.complex64)
+    check_case(x, [1], True, 1.0, [-(n // 2)], [n // 2])
+    check_case(x, [1], False, 1.0 / n, [12345], None)
+
+
+def test_not_fusable_writes_nothing_and_frontend_falls_back():
+    """2^20 complex64 with an input roll of one element: not a whole column
+    of the four-step plan, so nothing is fused."""
+    import torch
+    n = 1 << 20
     rs = np.random.RandomState(4)
     x = crandn(rs, (2, n), np.complex64)
     t = to_gpu(x)
     out = torch.full_like(t, 7.0)
-    assert not R.c2c_shifted(t, out, np.array([1]), True, 1.0, [-(n // 2)], [n // 2])
+    assert not R.c2c_shifted(t, out, np.array([1]), True, 1.0, [1], None)
     torch.cuda.synchronize()
     assert bool((out == 7.0).all())
-    got = F.centered_fftn(t, axes=[1]).cpu().numpy()
-    want = np.fft.fftshift(np.fft.fft(np.fft.ifftshift(x.astype(np.complex128), axes=1), axis=1),
+    got = F.fftn_shifted(t, axes=[1], shift_in=[1], shift_out="fftshift").cpu().numpy()
+    want = np.fft.fftshift(np.fft.fft(np.roll(x.astype(np.complex128), 1, axis=1), axis=1),
                            axes=1)
     assert rel_l2(got, want) <= 1e-6
 
