@@ -2,6 +2,10 @@
 // instructions: one address register per thread, per-element offsets as
 // scalar operands, and the resource's range check bounding every access to
 // [base, base + bytes) (out-of-range loads read zeros, stores are dropped).
+// Both carry the non-temporal cache policy: the pipelined passes read each
+// element once and write it once, and streaming them past the caches'
+// normal replacement measured faster (64 x 2^20 c64: 0.444 -> 0.422 ms;
+// pass A 234 -> 222 us, pass B 208 -> 200 us; lines of 4096 unchanged).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,6 +20,8 @@ namespace bufio {
 
 typedef __attribute__((ext_vector_type(2))) unsigned u2;
 
+constexpr int kNonTemporal = 2;  // cache-policy operand of the buffer builtins: nt
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, int(bytes),
                                            0x00020000);
@@ -23,7 +29,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint3
 
 __device__ __forceinline__ dev::C2<float> bload(__amdgpu_buffer_rsrc_t r, uint32_t voff,
                                                 uint32_t soff) {
-  const u2 v = __builtin_amdgcn_raw_buffer_load_b64(r, int(voff), int(soff), 0);
+  const u2 v = __builtin_amdgcn_raw_buffer_load_b64(r, int(voff), int(soff), kNonTemporal);
   // components through scalar temporaries: a bit cast of a vector element
   // expression itself would read the vector's first element
   const unsigned lo = v.x, hi = v.y;
@@ -35,7 +41,7 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t voff, 
   u2 v;
   v[0] = __builtin_bit_cast(unsigned, x.x);
   v[1] = __builtin_bit_cast(unsigned, x.y);
-  __builtin_amdgcn_raw_buffer_store_b64(v, r, int(voff), int(soff), 0);
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, int(voff), int(soff), kNonTemporal);
 }
 
 // Largest scalar offset (floats) reached from a.in (in) or a.out by a line
