@@ -47,9 +47,9 @@ MODES = ("fp32", "mfma_fp16", "mfma_bf16")
 # same array in the same run (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000);
 # None where not measured for a kernel.
 TRAFFIC = {
-    "k_pass1024 passA n=1024 c64 strided": 1074017619,  # 1.0003 x the 1,073,741,824 algorithmic
-    "k_pass1024 passB n=1024 c64": 1073876148,          # 1.0001 x
-    "k_pipe4096 c2c n=4096 c64": 1073936619,            # 1.0002 x 1,073,741,824
+    "k_pass1024 passA n=1024 c64 strided": 1074434252,  # 1.0006 x the 1,073,741,824 algorithmic
+    "k_pass1024 passB n=1024 c64": 1074200323,          # 1.0004 x
+    "k_pipe4096 c2c n=4096 c64": 1074480235,            # 1.0007 x 1,073,741,824
 }
 
 
