@@ -24,6 +24,7 @@
 #include <string>
 #include <vector>
 
+#include "bufio.hpp"
 #include "passes.hpp"
 
 namespace rfh {
@@ -32,8 +33,8 @@ using namespace dev;
 
 thread_local const char *t_last_family = "k_lines";  // kernel family of the last launch
 
-template <typename T, int OP, bool VIN, bool VOUT>
-__global__ void __launch_bounds__(kThreads) k_lines(const LineArgs<T> a) {
+template <typename T, int OP, bool VIN, bool VOUT, int NT = kThreads>
+__global__ void __launch_bounds__(NT) k_lines(const LineArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int64_t s_oi[kMaxLinesPerGroup], s_oo[kMaxLinesPerGroup];
   __shared__ int s_aux[kMaxLinesPerGroup];
@@ -112,14 +113,34 @@ void allow_lds(const void *fn, size_t bytes) {
   g_attr_done.insert({dev, fn});
 }
 
+// Threads of the wide line workgroups: 1024 for complex64 (the float line
+// kernels stay within the 128 registers that allows), 512 for complex128
+// (within 256).
+template <typename T>
+constexpr int kWideThreads = sizeof(T) == 4 ? 1024 : 512;
+
 template <typename T, int OP, bool VIN, bool VOUT>
 void launch_one(const LineArgs<T> &a, size_t lds, hipStream_t s) {
-  const void *fn = reinterpret_cast<const void *>(&k_lines<T, OP, VIN, VOUT>);
-  allow_lds(fn, lds);
   const int64_t groups = (a.nlines + a.L - 1) / a.L;
   if (groups > 0x7fffffff / kThreads) RFH_THROW(3, "batch too large for one launch");
-  hipLaunchKernelGGL((k_lines<T, OP, VIN, VOUT>), dim3(unsigned(groups)), dim3(kThreads), lds,
-                     s, a);
+  bool wide = false;
+  if constexpr (OP == OP_C2C || OP == OP_BLUE) {
+    // At most one group per CU, the launch is bound by each group's stage
+    // latency (four waves per CU at the narrow width): a wider group runs a
+    // stage's butterflies in fewer rounds, as long as every thread still has
+    // two elements of the group.
+    wide = groups <= bufio::device_cus() && int64_t(a.L) * a.ld >= 2 * kWideThreads<T>;
+    if (wide) {
+      allow_lds(reinterpret_cast<const void *>(&k_lines<T, OP, VIN, VOUT, kWideThreads<T>>), lds);
+      hipLaunchKernelGGL((k_lines<T, OP, VIN, VOUT, kWideThreads<T>>), dim3(unsigned(groups)),
+                         dim3(kWideThreads<T>), lds, s, a);
+    }
+  }
+  if (!wide) {
+    allow_lds(reinterpret_cast<const void *>(&k_lines<T, OP, VIN, VOUT>), lds);
+    hipLaunchKernelGGL((k_lines<T, OP, VIN, VOUT>), dim3(unsigned(groups)), dim3(kThreads), lds,
+                       s, a);
+  }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     RFH_THROW(2, std::string("line kernel launch (") + std::to_string(groups) + " groups, " +
