@@ -38,6 +38,25 @@ def test_c2c_lengths(n, dtype):
     check("c2c", x, x, np.array([1]), False, 1.0 / n)
 
 
+@pytest.mark.parametrize("n,dtype", [(2187, np.complex64), (4099, np.complex64),
+                                     (8232, np.complex64), (1536, np.complex128),
+                                     (2187, np.complex128), (4116, np.complex128)])
+def test_wide_and_narrow_line_groups(n, dtype):
+    """A launch of at most one line group per CU runs the line kernel with
+    1024 (complex64) / 512 (complex128) threads per group, a larger batch
+    with 256: 7 lines take the first, 300 lines (more groups than the
+    MI355X's 256 CUs) the second.  Both against the oracle, and the 7 lines
+    of both launches against each other."""
+    rs = np.random.RandomState(n + 17)
+    x = crandn(rs, (300, n), dtype)
+    check("c2c", x, x, np.array([1]), True, 1.0)
+    check("c2c", x[:7].copy(), x[:7], np.array([1]), True, 1.0)
+    narrow = on_gpu(R, "c2c", x, x, np.array([1]), True, 1.0)
+    wide = on_gpu(R, "c2c", x[:7].copy(), x[:7], np.array([1]), True, 1.0)
+    e = rel_l2(wide, narrow[:7])
+    assert e <= tol_of(dtype), f"wide vs narrow groups: rel-L2 {e:.3e}"
+
+
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 def test_c2c_two_pass_long(dtype):
     rs = np.random.RandomState(7)
