@@ -57,6 +57,18 @@ def test_wide_and_narrow_line_groups(n, dtype):
     assert e <= tol_of(dtype), f"wide vs narrow groups: rel-L2 {e:.3e}"
 
 
+@pytest.mark.parametrize("shape,dtype", [((1000, 24), np.complex64), ((243, 40), np.complex64),
+                                         ((600, 12), np.complex128), ((1000, 6), np.complex128)])
+def test_wide_line_groups_interleaved(shape, dtype):
+    """Lines along axis 0 (interleaved in memory, several adjacent lines per
+    group): a few groups of at least two elements per thread, so the
+    wide-group launch with the line-fastest loads and stores."""
+    rs = np.random.RandomState(shape[0] + shape[1])
+    x = crandn(rs, shape, dtype)
+    check("c2c", x, x, np.array([0]), True, 1.0)
+    check("c2c", x, x, np.array([0]), False, 1.0 / shape[0])
+
+
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 def test_c2c_two_pass_long(dtype):
     rs = np.random.RandomState(7)
