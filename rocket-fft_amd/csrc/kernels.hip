@@ -60,8 +60,8 @@ int ilog2(int v) {
 int choose_lines(int ld, size_t elem, int64_t nlines, bool interleaved) {
   // interleaved lines want as many adjacent lines as LDS holds (row
   // segments); contiguous lines leave room for several groups per CU
-  const size_t budget = interleaved ? lds_limit() - 8192
-                                    : std::min<size_t>(48 * 1024, lds_limit() - 8192);
+  const size_t budget = interleaved ? lds_limit() - kLineStaticLds
+                                    : std::min<size_t>(48 * 1024, lds_limit() - kLineStaticLds);
   int L = 1;
   while (L * 2 <= kMaxLinesPerGroup && size_t(2) * (L * 2) * ld * elem <= budget &&
          int64_t(L) < nlines)
@@ -338,7 +338,7 @@ void run_pass(const PassSpec &ps, const LineSet &g, hipStream_t s) {
   LineArgs<T> a = make_line_args<T>(ps, g);
   const size_t elem = 2 * sizeof(T);
   const size_t lds = size_t(2) * a.L * a.ld * elem;
-  if (lds + 8192 > lds_limit()) RFH_THROW(3, "line does not fit in LDS");
+  if (lds + kLineStaticLds > lds_limit()) RFH_THROW(3, "line does not fit in LDS");
   bool vin, vout;
   if (ps.real_in)  // real pairs: 2-wide loads when the line is contiguous
     vin = g.esi == 1 && pairs_aligned(g.in, sizeof(T), 0, g.bsi, g.nb);

@@ -239,7 +239,7 @@ size_t lds_limit() {
 }
 
 // Static LDS of the line kernels (per-line offsets) is reserved first.
-static size_t lds_dynamic_budget() { return lds_limit() - 8192; }
+static size_t lds_dynamic_budget() { return lds_limit() - kLineStaticLds; }
 
 uint64_t direct_max(bool dbl, int lines, size_t budget) {
   const size_t elem = dbl ? 16 : 8;

@@ -22,6 +22,12 @@ constexpr int kMaxBatchDims = 10;
 constexpr int kMaxDims = 32;
 constexpr int kMaxLinesPerGroup = 256;
 constexpr int kThreads = 256;
+// LDS kept back from the line kernels' dynamic buffers for their static
+// arrays: the per-line offset slots, 8 + 8 + 4 bytes for each of up to
+// kMaxLinesPerGroup lines (5120 B), rounded up.  The launch checks the
+// kernel's actual static size against the device limit.
+constexpr size_t kLineStaticLds = 6144;
+static_assert(kLineStaticLds >= kMaxLinesPerGroup * (8 + 8 + 4), "line kernel static LDS");
 
 // ------------------------------------------------------------------ errors
 

@@ -57,6 +57,20 @@ def test_wide_and_narrow_line_groups(n, dtype):
     assert e <= tol_of(dtype), f"wide vs narrow groups: rel-L2 {e:.3e}"
 
 
+@pytest.mark.parametrize("n,dtype", [(4913, np.complex128), (4928, np.complex128),
+                                     (4929, np.complex128), (9856, np.complex64),
+                                     (9857, np.complex64)])
+@pytest.mark.parametrize("lines", [3, 300])
+def test_one_pass_limit(n, dtype, lines):
+    """Lengths at the one-LDS-pass limit (two line buffers beside the line
+    kernel's 5 KB of static LDS in 160 KB: 4928 complex128, 9856 complex64
+    elements) and one past it (four-step passes), few and many lines."""
+    rs = np.random.RandomState(n + lines)
+    x = crandn(rs, (lines, n), dtype)
+    check("c2c", x, x, np.array([1]), True, 1.0)
+    check("c2c", x, x, np.array([1]), False, 1.0 / n)
+
+
 @pytest.mark.parametrize("shape,dtype", [((1000, 24), np.complex64), ((243, 40), np.complex64),
                                          ((600, 12), np.complex128), ((1000, 6), np.complex128)])
 def test_wide_line_groups_interleaved(shape, dtype):
