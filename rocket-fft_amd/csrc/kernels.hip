@@ -128,8 +128,11 @@ void launch_one(const LineArgs<T> &a, size_t lds, hipStream_t s) {
     // At most one group per CU, the launch is bound by each group's stage
     // latency (four waves per CU at the narrow width): a wider group runs a
     // stage's butterflies in fewer rounds, as long as every thread still has
-    // two elements of the group.
-    wide = groups <= bufio::device_cus() && int64_t(a.L) * a.ld >= 2 * kWideThreads<T>;
+    // two (complex64) or four (complex128) elements of the group; below that
+    // the wider barriers cost more than they save (256 lines of complex128
+    // 1536: 10.1 us narrow, 10.8 us wide).
+    constexpr int64_t kMinElems = (sizeof(T) == 4 ? 2 : 4) * kWideThreads<T>;
+    wide = groups <= bufio::device_cus() && int64_t(a.L) * a.ld >= kMinElems;
     if (wide) {
       allow_lds(reinterpret_cast<const void *>(&k_lines<T, OP, VIN, VOUT, kWideThreads<T>>), lds);
       hipLaunchKernelGGL((k_lines<T, OP, VIN, VOUT, kWideThreads<T>>), dim3(unsigned(groups)),

@@ -39,14 +39,15 @@ def test_c2c_lengths(n, dtype):
 
 
 @pytest.mark.parametrize("n,dtype", [(2187, np.complex64), (4099, np.complex64),
-                                     (8232, np.complex64), (1536, np.complex128),
-                                     (2187, np.complex128), (4116, np.complex128)])
+                                     (8232, np.complex64), (2187, np.complex128),
+                                     (4116, np.complex128), (4913, np.complex128)])
 def test_wide_and_narrow_line_groups(n, dtype):
     """A launch of at most one line group per CU runs the line kernel with
-    1024 (complex64) / 512 (complex128) threads per group, a larger batch
-    with 256: 7 lines take the first, 300 lines (more groups than the
-    MI355X's 256 CUs) the second.  Both against the oracle, and the 7 lines
-    of both launches against each other."""
+    1024 (complex64, from 2048 elements per group) / 512 (complex128, from
+    2048) threads per group, a larger batch with 256: 7 lines take the
+    first, 300 lines (more groups than the MI355X's 256 CUs) the second.
+    Both against the oracle, and the 7 lines of both launches against each
+    other."""
     rs = np.random.RandomState(n + 17)
     x = crandn(rs, (300, n), dtype)
     check("c2c", x, x, np.array([1]), True, 1.0)
@@ -72,11 +73,11 @@ def test_one_pass_limit(n, dtype, lines):
 
 
 @pytest.mark.parametrize("shape,dtype", [((1000, 24), np.complex64), ((243, 40), np.complex64),
-                                         ((600, 12), np.complex128), ((1000, 6), np.complex128)])
+                                         ((600, 12), np.complex128), ((1200, 6), np.complex128)])
 def test_wide_line_groups_interleaved(shape, dtype):
     """Lines along axis 0 (interleaved in memory, several adjacent lines per
-    group): a few groups of at least two elements per thread, so the
-    wide-group launch with the line-fastest loads and stores."""
+    group): a few groups of at least 2048 elements, so the wide-group
+    launch with the line-fastest loads and stores."""
     rs = np.random.RandomState(shape[0] + shape[1])
     x = crandn(rs, shape, dtype)
     check("c2c", x, x, np.array([0]), True, 1.0)
